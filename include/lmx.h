@@ -1,0 +1,205 @@
+/*
+ * lmx.h -- C ABI of the MI355X (gfx950) dense-solve library.
+ *
+ * This is the drop-in boundary for the dense-solve path of linear-massiv
+ * (Numeric.LinearAlgebra.Massiv): a Haskell `foreign import ccall` module
+ * binds these symbols and keeps the library's public signatures unchanged.
+ * Each entry point names the reference function it replaces (paths relative
+ * to the reference repository).
+ *
+ * Conventions
+ *   - Matrices are ROW-MAJOR, as massiv's P arrays are
+ *     (src/Numeric/LinearAlgebra/Massiv/Types.hs:90); `ld*` is the row
+ *     stride in elements (>= the column count).
+ *   - Right-hand sides are an n x nrhs row-major block; nrhs = 1 with
+ *     ldb = 1 is a massiv Vector.
+ *   - Pivots use the LAPACK swap-list convention, 0-based: at step k rows k
+ *     and ipiv[k] were exchanged.  This is the (k, pivRow) list that
+ *     luSolveP builds (src/.../Solve/LU.hs:276-284, applied at :493-507);
+ *     the permutation vector of the generic `lu` (src/.../Solve/LU.hs:121)
+ *     follows by applying the swaps to 0..n-1 in ascending k.
+ *   - Return value: LMX_OK (0); > 0 is a LAPACK-style `info` (1-based index
+ *     of the first exactly-zero pivot, or of the first Cholesky column whose
+ *     pivot is not positive) -- the computation still runs to the end and
+ *     its Inf/NaN output is returned, as the reference does not stop either;
+ *     < 0 is an error code (below).  lmx_last_error() describes the most
+ *     recent error of the calling thread.
+ *   - Host-pointer functions (no suffix) are synchronous: they copy to HBM,
+ *     compute and copy back.  `_dev` functions take device pointers and a
+ *     hipStream_t (passed as void*, NULL = HIP's default stream) and are
+ *     stream-ordered; their `info` goes to a device int32 so no host
+ *     synchronisation is needed.
+ *   - Every function is thread-safe.
+ */
+#ifndef LMX_H
+#define LMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMX_OK 0
+#define LMX_ERR_ARG (-1)          /* invalid argument (size, stride, flag) */
+#define LMX_ERR_DEVICE (-2)       /* HIP runtime / device failure */
+#define LMX_ERR_NOMEM (-3)        /* device memory exhausted */
+#define LMX_ERR_NOT_CONVERGED (-4) /* refinement did not reach the target */
+#define LMX_ERR_BUDGET (-5)       /* request larger than the HBM budget */
+#define LMX_ERR_INTERNAL (-6)
+
+/* Factor precisions for the mixed-precision solvers. */
+#define LMX_PREC_F64 0
+#define LMX_PREC_F32 1
+#define LMX_PREC_BF16 2
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Select and initialise a device (idempotent).  Replaces nothing in the
+ * reference, which has no device. */
+int lmx_init(int device);
+/* Release device resources held by the library. */
+int lmx_finalize(void);
+/* Thread-local description of the last error (never NULL). */
+const char *lmx_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int lmx_version(void);
+
+/* ---- GEMM --------------------------------------------------------------- */
+
+/* C = alpha op(A) op(B) + beta C, op(X) = X ('N') or X^T ('T'); op(A) is
+ * m x k, op(B) is k x n.  beta = 0 does not read C.
+ * Replaces gemm (src/.../BLAS/Level3.hs:102-131) and, with alpha = 1,
+ * beta = 0, matMul / matMulP / matMulPPar (:144-147, :183-207, :214-294). */
+int lmx_dgemm(char transa, char transb, int64_t m, int64_t n, int64_t k,
+              double alpha, const double *A, int64_t lda, const double *B,
+              int64_t ldb, double beta, double *C, int64_t ldc);
+int lmx_dgemm_dev(void *stream, char transa, char transb, int64_t m,
+                  int64_t n, int64_t k, double alpha, const double *dA,
+                  int64_t lda, const double *dB, int64_t ldb, double beta,
+                  double *dC, int64_t ldc);
+/* Single precision on FP32 MFMA; matMul at e = Float (the generic path
+ * src/.../BLAS/Level3.hs:150-181). */
+int lmx_sgemm_dev(void *stream, char transa, char transb, int64_t m,
+                  int64_t n, int64_t k, float alpha, const float *dA,
+                  int64_t lda, const float *dB, int64_t ldb, float beta,
+                  float *dC, int64_t ldc);
+int lmx_sgemm(char transa, char transb, int64_t m, int64_t n, int64_t k,
+              float alpha, const float *A, int64_t lda, const float *B,
+              int64_t ldb, float beta, float *C, int64_t ldc);
+
+/* ---- LU ----------------------------------------------------------------- */
+
+/* In-place LU with partial pivoting of the n x n matrix A: unit L below the
+ * diagonal, U on and above it; ipiv[k] = pivot row of step k (n entries).
+ * Replaces the factorisation of luSolveP (panelLUFactor,
+ * src/.../Solve/LU.hs:276-284, :311-357) and of lu (:121-167). */
+int lmx_dgetrf(int64_t n, double *A, int64_t lda, int32_t *ipiv);
+int lmx_dgetrf_dev(void *stream, int64_t n, double *dA, int64_t lda,
+                   int32_t *d_ipiv, int32_t *d_info);
+/* Solve A X = B with the factors of lmx_dgetrf; B (n x nrhs) is
+ * overwritten by X.  Replaces the solve phase of luSolveP
+ * (src/.../Solve/LU.hs:294-303: applyPivotsForward,
+ * rawForwardSubUnitPackedSIMD, rawBackSubPackedSIMD) for nrhs columns. */
+int lmx_dgetrs(int64_t n, int64_t nrhs, const double *LU, int64_t lda,
+               const int32_t *ipiv, double *B, int64_t ldb);
+int lmx_dgetrs_dev(void *stream, int64_t n, int64_t nrhs, const double *dLU,
+                   int64_t lda, const int32_t *d_ipiv, double *dB,
+                   int64_t ldb);
+/* Factor and solve: A is overwritten by its factors, B by X.
+ * Replaces luSolveP / luSolve (src/.../Solve/LU.hs:265-304, :240-259). */
+int lmx_dgesv(int64_t n, int64_t nrhs, double *A, int64_t lda, int32_t *ipiv,
+              double *B, int64_t ldb);
+int lmx_dgesv_dev(void *stream, int64_t n, int64_t nrhs, double *dA,
+                  int64_t lda, int32_t *d_ipiv, double *dB, int64_t ldb,
+                  int32_t *d_info);
+
+/* ---- Cholesky ----------------------------------------------------------- */
+
+/* In-place lower Cholesky A = G G^T reading the lower triangle only; the
+ * strict upper triangle is left as given.  Replaces the factorisation of
+ * choleskySolveP (src/.../Solve/Cholesky.hs:246-256, panelCholFactor
+ * :279-321) and cholesky (:114-147, whose upper triangle is zero: the
+ * binding zeroes it). */
+int lmx_dpotrf(int64_t n, double *A, int64_t lda);
+int lmx_dpotrf_dev(void *stream, int64_t n, double *dA, int64_t lda,
+                   int32_t *d_info);
+/* Solve G G^T X = B with the lower factor.  Replaces the solve phase of
+ * choleskySolveP (src/.../Solve/Cholesky.hs:258-272). */
+int lmx_dpotrs(int64_t n, int64_t nrhs, const double *G, int64_t lda,
+               double *B, int64_t ldb);
+int lmx_dpotrs_dev(void *stream, int64_t n, int64_t nrhs, const double *dG,
+                   int64_t lda, double *dB, int64_t ldb);
+/* Factor and solve.  Replaces choleskySolveP / choleskySolve
+ * (src/.../Solve/Cholesky.hs:241-273, :229-235). */
+int lmx_dposv(int64_t n, int64_t nrhs, double *A, int64_t lda, double *B,
+              int64_t ldb);
+int lmx_dposv_dev(void *stream, int64_t n, int64_t nrhs, double *dA,
+                  int64_t lda, double *dB, int64_t ldb, int32_t *d_info);
+
+/* ---- triangular solves -------------------------------------------------- */
+
+/* Solve op(T) X = B in place (B is n x nrhs).  uplo 'L'/'U' selects the
+ * triangle of T that is read, trans 'N'/'T' selects op, diag 'U' treats the
+ * diagonal as ones without reading it, 'N' divides by it.
+ * Replaces forwardSub (uplo L, diag N), backSub (U, N), forwardSubUnit
+ * (L, U) and backSubUnit (U, U) of src/.../Solve/Triangular.hs:90-107,
+ * :136-153, :187-201, :235-249, and the packed substitutions
+ * src/.../Internal/Kernel.hs:2205-2430. */
+int lmx_dtrsm(char uplo, char trans, char diag, int64_t n, int64_t nrhs,
+              const double *T, int64_t ldt, double *B, int64_t ldb);
+int lmx_dtrsm_dev(void *stream, char uplo, char trans, char diag, int64_t n,
+                  int64_t nrhs, const double *dT, int64_t ldt, double *dB,
+                  int64_t ldb);
+
+/* ---- mixed precision with FP64 iterative refinement -------------------- */
+
+/* Outcome of a refined solve. */
+typedef struct lmx_refine_info {
+    int32_t iterations;      /* refinement steps taken */
+    int32_t converged;       /* 1 if the FP64 backward-error target was met */
+    int32_t fell_back;       /* 1 if the solve fell back to an FP64 factor */
+    int32_t info;            /* factorisation info of the final factor */
+    double backward_error;   /* max over columns of ||b-Ax||_inf /
+                                (||A||_inf ||x||_inf + ||b||_inf) */
+} lmx_refine_info;
+
+/* Solve A X = B to FP64 accuracy: factor a copy of A at `precision`
+ * (LMX_PREC_F32 or LMX_PREC_BF16) on MFMA with FP32 accumulation, then
+ * refine in FP64 (residual, correction, convergence test on the device)
+ * until the normwise backward error is <= 10 n eps or stagnates; falls back
+ * to the FP64 factorisation if it does not converge.  A and B are not
+ * modified.  No reference counterpart: pinned by the FP64 oracle. */
+int lmx_dgesv_mixed(int precision, int64_t n, int64_t nrhs, const double *A,
+                    int64_t lda, const double *B, int64_t ldb, double *X,
+                    int64_t ldx, lmx_refine_info *info);
+int lmx_dgesv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs,
+                        const double *dA, int64_t lda, const double *dB,
+                        int64_t ldb, double *dX, int64_t ldx,
+                        lmx_refine_info *info);
+int lmx_dposv_mixed(int precision, int64_t n, int64_t nrhs, const double *A,
+                    int64_t lda, const double *B, int64_t ldb, double *X,
+                    int64_t ldx, lmx_refine_info *info);
+int lmx_dposv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs,
+                        const double *dA, int64_t lda, const double *dB,
+                        int64_t ldb, double *dX, int64_t ldx,
+                        lmx_refine_info *info);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* When enabled, the library brackets every launch of the named kernel
+ * family with HIP events on the launch stream and accumulates the elapsed
+ * time.  Families: "gemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
+ * "syrk", "getrs", "refine".  No reference counterpart. */
+int lmx_kernel_timing(int enable);
+int lmx_kernel_timing_reset(void);
+/* Count of timed launches and their total milliseconds for one family
+ * (synchronises the device to read the events). */
+int lmx_kernel_timing_get(const char *family, int64_t *count,
+                          double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LMX_H */

@@ -1,0 +1,222 @@
+// Lower Cholesky factorisation (right-looking, blocked) and solve, for
+// row-major matrices.  Only the lower triangle is read or written.
+//
+// potrf_blocked(nb): for each diagonal block of width nb:
+//   - factor the block (recursively with nb = 64, then the one-workgroup
+//     LDS kernel potrf_diag_kernel for blocks of order <= 64);
+//   - L21 = A21 L11^{-T} (right-side blocked TRSM);
+//   - A22 -= L21 L21^T on the lower-triangle-only MFMA SYRK.
+// The reference's left-looking panelCholFactor (src/.../Solve/Cholesky.hs:
+// 279-321) computes the same factor; the right-looking order keeps the
+// large update a single wide MFMA launch per block column.
+// A pivot that is not > 0 (including NaN) sets info to its 1-based column
+// the first time; the factorisation continues and propagates NaN, as the
+// reference's sqrt of a negative number does.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace lmx {
+
+namespace {
+
+constexpr int DIAG = 64;
+
+// In-place lower Cholesky of an nb x nb block (nb <= 64) in LDS.
+template <class T>
+__global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
+                                                         int32_t *__restrict__ info) {
+    __shared__ T L[DIAG][DIAG + 1];
+    __shared__ int s_bad;
+    const int t = threadIdx.x;
+    for (int e = t; e < nb * nb; e += 256) {
+        int r = e / nb, c = e % nb;
+        L[r][c] = (c <= r) ? A[(int64_t)r * lda + c] : T(0);
+    }
+    if (t == 0) s_bad = 0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (t == 0) {
+            T d = L[j][j];
+            if (!(d > T(0)) && s_bad == 0) s_bad = col0 + j + 1;
+            L[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        const T djj = L[j][j];
+        for (int i = j + 1 + t; i < nb; i += 256) L[i][j] = L[i][j] / djj;
+        __syncthreads();
+        // trailing update of the lower triangle: L[i][c] -= L[i][j] L[c][j], j < c <= i
+        const int w = nb - j - 1;
+        for (int e = t; e < w * w; e += 256) {
+            int i = j + 1 + e / w, c = j + 1 + e % w;
+            if (c <= i) L[i][c] = fma(-L[i][j], L[c][j], L[i][c]);
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < nb * nb; e += 256) {
+        int r = e / nb, c = e % nb;
+        if (c <= r) A[(int64_t)r * lda + c] = L[r][c];
+    }
+    if (t == 0 && s_bad != 0 && *info == 0) *info = s_bad;
+}
+
+template <class T>
+void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, int32_t *d_info, int64_t nb) {
+    for (int64_t k0 = 0; k0 < n; k0 += nb) {
+        const int64_t kb = std::min(nb, n - k0);
+        T *A11 = A + k0 * lda + k0;
+        if (kb <= DIAG) {
+            TimedRegion tr("potrf_diag", s);
+            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)kb, A11, lda,
+                               (int)(col0 + k0), d_info);
+            LMX_LAUNCH_CHECK();
+        } else {
+            potrf_blocked<T>(s, kb, A11, lda, col0 + k0, d_info, DIAG);
+        }
+        const int64_t rest = n - k0 - kb;
+        if (rest > 0) {
+            T *A21 = A11 + kb * lda;
+            trsm_right<T>(s, true, true, false, rest, kb, A11, lda, A21, lda);
+            syrk_lower(s, rest, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
+        }
+    }
+}
+
+}  // namespace
+
+template <class T>
+void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info) {
+    if (n < 0 || lda < std::max<int64_t>(1, n)) fail_arg("potrf: bad dimension or stride");
+    if (n > INT32_MAX / 2) fail_arg("potrf: n too large");
+    clear_info(s, d_info);
+    if (n == 0) return;
+    potrf_blocked<T>(s, n, A, lda, 0, d_info, 128);
+}
+
+template <class T>
+void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb) {
+    if (n == 0 || nrhs == 0) return;
+    trsm_left<T>(s, true, false, false, n, nrhs, L, lda, B, ldb);
+    trsm_left<T>(s, true, true, false, n, nrhs, L, lda, B, ldb);
+}
+
+template void potrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *);
+template void potrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *);
+template void potrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, double *, int64_t);
+template void potrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, float *, int64_t);
+
+}  // namespace lmx
+
+using namespace lmx;
+
+namespace {
+
+size_t extent(int64_t rows, int64_t cols, int64_t ld) {
+    return (rows > 0 && cols > 0) ? (size_t)((rows - 1) * ld + cols) * sizeof(double) : 0;
+}
+
+void check(int64_t n, int64_t lda, const char *who) {
+    if (n < 0) fail_arg(std::string(who) + ": negative dimension");
+    if (lda < std::max<int64_t>(1, n)) fail_arg(std::string(who) + ": lda too small");
+}
+
+void check_rhs(int64_t nrhs, int64_t ldb, const char *who) {
+    if (nrhs < 0) fail_arg(std::string(who) + ": negative nrhs");
+    if (ldb < std::max<int64_t>(1, nrhs)) fail_arg(std::string(who) + ": ldb too small");
+}
+
+int read_info(hipStream_t s, const int32_t *d_info) {
+    int32_t info = 0;
+    LMX_HIP(hipMemcpyAsync(&info, d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LMX_HIP(hipStreamSynchronize(s));
+    return info;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmx_dpotrf_dev(void *stream, int64_t n, double *dA, int64_t lda, int32_t *d_info) {
+    return guarded([&] {
+        check(n, lda, "potrf");
+        potrf<double>(resolve_stream(stream), n, dA, lda, d_info);
+        return LMX_OK;
+    });
+}
+
+int lmx_dpotrs_dev(void *stream, int64_t n, int64_t nrhs, const double *dG, int64_t lda, double *dB,
+                   int64_t ldb) {
+    return guarded([&] {
+        check(n, lda, "potrs");
+        check_rhs(nrhs, ldb, "potrs");
+        potrs<double>(resolve_stream(stream), n, nrhs, dG, lda, dB, ldb);
+        return LMX_OK;
+    });
+}
+
+int lmx_dposv_dev(void *stream, int64_t n, int64_t nrhs, double *dA, int64_t lda, double *dB, int64_t ldb,
+                  int32_t *d_info) {
+    return guarded([&] {
+        check(n, lda, "posv");
+        check_rhs(nrhs, ldb, "posv");
+        hipStream_t s = resolve_stream(stream);
+        potrf<double>(s, n, dA, lda, d_info);
+        potrs<double>(s, n, nrhs, dA, lda, dB, ldb);
+        return LMX_OK;
+    });
+}
+
+int lmx_dpotrf(int64_t n, double *A, int64_t lda) {
+    return guarded([&] {
+        check(n, lda, "potrf");
+        if (n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda);
+        double *dA = (double *)staging(c, 0, ab);
+        int32_t *di = (int32_t *)staging(c, 1, 64);
+        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        potrf<double>(c.stream, n, dA, lda, di);
+        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
+        return read_info(c.stream, di);
+    });
+}
+
+int lmx_dpotrs(int64_t n, int64_t nrhs, const double *G, int64_t lda, double *B, int64_t ldb) {
+    return guarded([&] {
+        check(n, lda, "potrs");
+        check_rhs(nrhs, ldb, "potrs");
+        if (n == 0 || nrhs == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda), bb = extent(n, nrhs, ldb);
+        double *dA = (double *)staging(c, 0, ab), *dB = (double *)staging(c, 1, bb);
+        LMX_HIP(hipMemcpyAsync(dA, G, ab, hipMemcpyHostToDevice, c.stream));
+        LMX_HIP(hipMemcpyAsync(dB, B, bb, hipMemcpyHostToDevice, c.stream));
+        potrs<double>(c.stream, n, nrhs, dA, lda, dB, ldb);
+        LMX_HIP(hipMemcpyAsync(B, dB, bb, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipStreamSynchronize(c.stream));
+        return LMX_OK;
+    });
+}
+
+int lmx_dposv(int64_t n, int64_t nrhs, double *A, int64_t lda, double *B, int64_t ldb) {
+    return guarded([&] {
+        check(n, lda, "posv");
+        check_rhs(nrhs, ldb, "posv");
+        if (n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda), bb = extent(n, nrhs, ldb);
+        double *dA = (double *)staging(c, 0, ab), *dB = (double *)staging(c, 1, bb);
+        int32_t *di = (int32_t *)staging(c, 2, 64);
+        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        if (bb) LMX_HIP(hipMemcpyAsync(dB, B, bb, hipMemcpyHostToDevice, c.stream));
+        potrf<double>(c.stream, n, dA, lda, di);
+        potrs<double>(c.stream, n, nrhs, dA, lda, dB, ldb);
+        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
+        if (bb) LMX_HIP(hipMemcpyAsync(B, dB, bb, hipMemcpyDeviceToHost, c.stream));
+        return read_info(c.stream, di);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Internal runtime pieces shared by the kernels and the C ABI: error
+// handling, the per-device context, stream-ordered scratch memory and the
+// optional per-kernel-family event timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <utility>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "lmx.h"
+
+namespace lmx {
+
+// An error raised inside the library; converted to a status code at the C
+// ABI boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char *what, const char *file, int line);
+[[noreturn]] inline void fail_arg(const std::string &m) { throw Error(LMX_ERR_ARG, m); }
+
+#define LMX_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t lmx_e_ = (call);                                          \
+        if (lmx_e_ != hipSuccess) ::lmx::throw_hip(lmx_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define LMX_LAUNCH_CHECK() LMX_HIP(hipGetLastError())
+
+void set_last_error(const std::string &m);
+
+// Runs f() and maps exceptions to status codes; f returns the status.
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return LMX_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return LMX_ERR_INTERNAL;
+    }
+}
+
+// Per-device state.
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;  // used by the host-pointer calls
+    std::mutex host_mu;            // serialises host-pointer calls (they share the staging buffers)
+    void *staging[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t staging_bytes[4] = {0, 0, 0, 0};
+    int cu_count = 0;
+};
+
+// Context of the current device (initialised on first use).
+Context &ctx();
+hipStream_t resolve_stream(void *s);
+// Grow-only device staging buffer `slot` of at least `bytes` (host calls only,
+// under host_mu).
+void *staging(Context &c, int slot, size_t bytes);
+
+// Stream-ordered scratch: allocated with hipMallocAsync on `s`, freed on the
+// same stream when the object goes out of scope.
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    Scratch(size_t bytes, hipStream_t st);
+    ~Scratch();
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Optional per-family kernel timing (lmx_kernel_timing).
+bool timing_enabled();
+struct TimedRegion {
+    int slot = -1;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TimedRegion(const char *family, hipStream_t st);
+    ~TimedRegion();
+};
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline bool is_trans(char t) {
+    if (t == 'N' || t == 'n') return false;
+    if (t == 'T' || t == 't' || t == 'C' || t == 'c') return true;
+    fail_arg(std::string("trans flag must be N or T, got '") + t + "'");
+}
+
+}  // namespace lmx

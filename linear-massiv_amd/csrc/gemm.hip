@@ -1,0 +1,381 @@
+// Row-major GEMM on CDNA4 matrix cores.
+//
+// C = alpha op(A) op(B) + beta C.  One workgroup of 256 threads (4 wave64s
+// in a 2 x 2 grid) computes a BM x BN tile of C; each wave owns a
+// (BM/2) x (BN/2) sub-tile made of 16 x 16 MFMA blocks held in accumulator
+// registers.  K advances in steps of BK through two LDS buffers: the next
+// step's global loads are issued into registers before the current step's
+// MFMAs and written to the other buffer after them, one barrier per step.
+//
+// MFMA 16x16x4 operands (f64 and f32 alike): lane l supplies
+// A[i = l%16][kk = l/16] and B[kk = l/16][j = l%16], so both operand tiles
+// live in LDS k-major (As[kk][i], Bs[kk][j]) and the 16 lanes of a
+// quarter-wave read 16 consecutive elements.  The four accumulator values of
+// lane l sit in column l%16 at rows 4*(l/16) + r for f32 but at rows
+// l/16 + 4*r for f64 (measured on gfx950); acc_row() encodes both.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace lmx {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct Acc;
+template <>
+struct Acc<double> {
+    typedef d4 type;
+};
+template <>
+struct Acc<float> {
+    typedef f4 type;
+};
+
+__device__ inline d4 mfma16x16x4(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4 mfma16x16x4(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Maps the linear workgroup id to a C tile.  Workgroups are dispatched
+// round-robin over the 8 XCDs; remapping gives each XCD a contiguous range
+// of ids, and the grouped order inside a range keeps GROUP_M tile-rows
+// sharing B panels in that XCD's L2.
+// With `lower` the grid enumerates only the tiles on or below the diagonal
+// of a square tile grid (tn <= tm), row by row.
+__device__ inline void tile_of(int tiles_m, int tiles_n, bool lower, int &tm, int &tn) {
+    int nwg = lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
+    int bid = blockIdx.x;
+    if ((nwg & 7) == 0) bid = (bid & 7) * (nwg >> 3) + (bid >> 3);
+    if (lower) {
+        int r = (int)((sqrtf(8.0f * (float)bid + 1.0f) - 1.0f) * 0.5f);
+        while (r * (r + 1) / 2 > bid) --r;
+        while ((r + 1) * (r + 2) / 2 <= bid) ++r;
+        tm = r;
+        tn = bid - r * (r + 1) / 2;
+        return;
+    }
+    const int GROUP_M = 8;
+    int per_group = GROUP_M * tiles_n;
+    int g = bid / per_group;
+    int first_m = g * GROUP_M;
+    int gsize = min(tiles_m - first_m, GROUP_M);
+    int in_g = bid - g * per_group;
+    tm = first_m + in_g % gsize;
+    tn = in_g / gsize;
+}
+
+// Row inside a 16 x 16 accumulator block of value r held by lane group lk.
+template <class T>
+__device__ inline int acc_row(int lk, int r) {
+    return sizeof(T) == 8 ? lk + 4 * r : 4 * lk + r;
+}
+
+template <class T, int BM, int BN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T alpha,
+                                                        const T *__restrict__ A, int64_t lda,
+                                                        const T *__restrict__ B, int64_t ldb,
+                                                        T beta, T *__restrict__ C, int64_t ldc,
+                                                        int tiles_m, int tiles_n, int lower) {
+    constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int MT = WM / 16, NT = WN / 16;
+    constexpr int LA = BM * BK / 256;  // A elements loaded per thread per step
+    constexpr int LB = BN * BK / 256;
+    static_assert(BM * BK % 256 == 0 && BN * BK % 256 == 0 && BK % 4 == 0, "tile shape");
+    typedef typename Acc<T>::type V;
+
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
+
+    int tm, tn;
+    tile_of(tiles_m, tiles_n, lower != 0, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    V acc[MT][NT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) acc[a][b] = V{0, 0, 0, 0};
+
+    T ra[LA], rb[LB];
+
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            int e = it * 256 + tid;
+            int i, kk;
+            if (TA) {
+                i = e % BM;
+                kk = e / BM;
+            } else {
+                kk = e % BK;
+                i = e / BK;
+            }
+            int gi = m0 + i, gk = k0 + kk;
+            T v = T(0);
+            if (gi < m && gk < k) v = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
+            ra[it] = v;
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            int e = it * 256 + tid;
+            int j, kk;
+            if (TB) {
+                kk = e % BK;
+                j = e / BK;
+            } else {
+                j = e % BN;
+                kk = e / BN;
+            }
+            int gj = n0 + j, gk = k0 + kk;
+            T v = T(0);
+            if (gj < n && gk < k) v = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
+            rb[it] = v;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            int e = it * 256 + tid;
+            int i, kk;
+            if (TA) {
+                i = e % BM;
+                kk = e / BM;
+            } else {
+                kk = e % BK;
+                i = e / BK;
+            }
+            As[buf][kk][i] = ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            int e = it * 256 + tid;
+            int j, kk;
+            if (TB) {
+                kk = e % BK;
+                j = e / BK;
+            } else {
+                j = e % BN;
+                kk = e / BN;
+            }
+            Bs[buf][kk][j] = rb[it];
+        }
+    };
+
+    const int KT = (k + BK - 1) / BK;
+    if (KT > 0) {
+        load(0);
+        store(0);
+        __syncthreads();
+    }
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < KT;
+        if (more) load((kt + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            T fa[MT], fb[NT];
+#pragma unroll
+            for (int a = 0; a < MT; ++a) fa[a] = As[cur][ks + lk][wm0 + a * 16 + lr];
+#pragma unroll
+            for (int b = 0; b < NT; ++b) fb[b] = Bs[cur][ks + lk][wn0 + b * 16 + lr];
+#pragma unroll
+            for (int a = 0; a < MT; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b) acc[a][b] = mfma16x16x4(fa[a], fb[b], acc[a][b]);
+        }
+        if (more) store(cur ^ 1);
+        __syncthreads();
+    }
+
+    // Epilogue: lane holds C[acc_row(lk, r)][lr] of each 16 x 16 block.
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            int gj = n0 + wn0 + b * 16 + lr;
+            if (gj >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int gi = m0 + wm0 + a * 16 + acc_row<T>(lk, r);
+                if (gi < m && (!lower || gj <= gi)) {
+                    T *c = C + (int64_t)gi * ldc + gj;
+                    T v = alpha * acc[a][b][r];
+                    if (beta != T(0)) v += beta * *c;
+                    *c = v;
+                }
+            }
+        }
+}
+
+template <class T, int BM, int BN, int BK>
+void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, const T *A,
+                 int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0) {
+    int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
+    dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n), block(256);
+#define LMX_GEMM_CASE(TA_, TB_)                                                              \
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, BK, TA_, TB_>), grid, block, 0, s, m, n, k, \
+                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, lower)
+    if (!ta && !tb)
+        LMX_GEMM_CASE(false, false);
+    else if (!ta && tb)
+        LMX_GEMM_CASE(false, true);
+    else if (ta && !tb)
+        LMX_GEMM_CASE(true, false);
+    else
+        LMX_GEMM_CASE(true, true);
+#undef LMX_GEMM_CASE
+    LMX_LAUNCH_CHECK();
+}
+
+template <class T>
+void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
+               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc) {
+    if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
+    if (m == 0 || n == 0) return;
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
+    TimedRegion tr("gemm", s);
+    // Large outputs: 128 x 128 tiles.  Smaller ones: 64 x 64 tiles so the
+    // launch still spreads over the 256 CUs.
+    int64_t big_tiles = cdiv(m, 128) * cdiv(n, 128);
+    if (big_tiles >= 512)
+        launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
+                                     C, ldc);
+    else
+        launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
+                                   C, ldc);
+}
+
+template <class T>
+void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
+               int64_t ldc) {
+    if (n <= 0) return;
+    if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
+    TimedRegion tr("syrk", s);
+    int64_t big_tiles = cdiv(n, 128) * (cdiv(n, 128) + 1) / 2;
+    if (big_tiles >= 512)
+        launch_tile<T, 128, 128, 16>(s, false, true, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
+                                     ldc, 1);
+    else
+        launch_tile<T, 64, 64, 16>(s, false, true, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
+                                   ldc, 1);
+}
+
+}  // namespace
+
+void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda, double beta,
+                double *C, int64_t ldc) {
+    syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, float beta,
+                float *C, int64_t ldc) {
+    syrk_impl<float>(s, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+          const double *A, int64_t lda, const double *B, int64_t ldb, double beta, double *C,
+          int64_t ldc) {
+    gemm_impl<double>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
+          const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+          int64_t ldc) {
+    gemm_impl<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+}  // namespace lmx
+
+using namespace lmx;
+
+namespace {
+
+void check_gemm_args(bool ta, bool tb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb,
+                     int64_t ldc) {
+    if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
+    int64_t acols = ta ? m : k, bcols = tb ? k : n;
+    if (lda < (acols > 1 ? acols : 1)) fail_arg("gemm: lda too small");
+    if (ldb < (bcols > 1 ? bcols : 1)) fail_arg("gemm: ldb too small");
+    if (ldc < (n > 1 ? n : 1)) fail_arg("gemm: ldc too small");
+}
+
+template <class T>
+int gemm_host(char transa, char transb, int64_t m, int64_t n, int64_t k, T alpha, const T *A,
+              int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc) {
+    return guarded([&] {
+        bool ta = is_trans(transa), tb = is_trans(transb);
+        check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
+        if (m == 0 || n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        int64_t arows = ta ? k : m, brows = tb ? n : k;
+        int64_t acols = ta ? m : k, bcols = tb ? k : n;
+        // exact extents: the last row ends at its last column, not at the stride
+        auto extent = [](int64_t rows, int64_t cols, int64_t ld) -> size_t {
+            return (rows > 0 && cols > 0) ? (size_t)((rows - 1) * ld + cols) * sizeof(T) : 0;
+        };
+        size_t abytes = extent(arows, acols, lda), bbytes = extent(brows, bcols, ldb),
+               cbytes = extent(m, n, ldc);
+        T *dA = (T *)staging(c, 0, abytes), *dB = (T *)staging(c, 1, bbytes),
+          *dC = (T *)staging(c, 2, cbytes);
+        if (abytes) LMX_HIP(hipMemcpyAsync(dA, A, abytes, hipMemcpyHostToDevice, c.stream));
+        if (bbytes) LMX_HIP(hipMemcpyAsync(dB, B, bbytes, hipMemcpyHostToDevice, c.stream));
+        if (beta != T(0)) LMX_HIP(hipMemcpyAsync(dC, C, cbytes, hipMemcpyHostToDevice, c.stream));
+        else if (ldc > n)  // keep the untouched padding of C intact on the way back
+            LMX_HIP(hipMemcpyAsync(dC, C, cbytes, hipMemcpyHostToDevice, c.stream));
+        gemm(c.stream, ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        LMX_HIP(hipMemcpyAsync(C, dC, cbytes, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipStreamSynchronize(c.stream));
+        return LMX_OK;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmx_dgemm(char transa, char transb, int64_t m, int64_t n, int64_t k, double alpha,
+              const double *A, int64_t lda, const double *B, int64_t ldb, double beta, double *C,
+              int64_t ldc) {
+    return gemm_host<double>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int lmx_sgemm(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+              const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+              int64_t ldc) {
+    return gemm_host<float>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int lmx_dgemm_dev(void *stream, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                  double alpha, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                  double beta, double *dC, int64_t ldc) {
+    return guarded([&] {
+        bool ta = is_trans(transa), tb = is_trans(transb);
+        check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
+        gemm(resolve_stream(stream), ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        return LMX_OK;
+    });
+}
+
+int lmx_sgemm_dev(void *stream, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                  float alpha, const float *dA, int64_t lda, const float *dB, int64_t ldb,
+                  float beta, float *dC, int64_t ldc) {
+    return guarded([&] {
+        bool ta = is_trans(transa), tb = is_trans(transb);
+        check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
+        gemm(resolve_stream(stream), ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        return LMX_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Internal (C++) entry points of the kernels, shared between the solver
+// drivers.  All matrices are row-major with an explicit row stride; all
+// launches are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lmx {
+
+// C = alpha op(A) op(B) + beta C on MFMA (FP64: v_mfma_f64_16x16x4_f64,
+// FP32: v_mfma_f32_16x16x4_f32).  beta == 0 never reads C.
+void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+          const double *A, int64_t lda, const double *B, int64_t ldb, double beta, double *C,
+          int64_t ldc);
+void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha,
+          const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+          int64_t ldc);
+
+// Lower triangle of C = alpha A A^T + beta C (A n x k); tiles strictly above
+// the diagonal are not launched and elements above it are not written.
+void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                double beta, double *C, int64_t ldc);
+void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, float beta,
+                float *C, int64_t ldc);
+
+// Triangular solve with many right-hand sides, in place on B (n x nrhs):
+// op(T) X = B, T read from the `lower` or upper triangle, `unit` diagonal
+// not read.
+template <class T>
+void trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int64_t n, int64_t nrhs,
+               const T *A, int64_t lda, T *B, int64_t ldb);
+// X op(T) = B in place on B (nrows x n).
+template <class T>
+void trsm_right(hipStream_t s, bool lower, bool trans, bool unit, int64_t nrows, int64_t n,
+                const T *A, int64_t lda, T *B, int64_t ldb);
+
+// LU with partial pivoting, in place; writes d_ipiv[0..n) and *d_info.
+template <class T>
+void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_info);
+// getrf that also writes the permutation vector (row i of PA is row
+// d_perm[i] of A) when d_perm is not null.
+template <class T>
+void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
+                int32_t *d_info);
+// Solve with getrf's factors and its permutation vector.
+template <class T>
+void getrs_perm(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda,
+                const int32_t *d_perm, T *B, int64_t ldb);
+// Solve with getrf's factors and swap list.
+template <class T>
+void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda,
+           const int32_t *d_ipiv, T *B, int64_t ldb);
+
+// Lower Cholesky, in place; writes *d_info.
+template <class T>
+void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info);
+template <class T>
+void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb);
+
+// Set *d_info = 0.
+void clear_info(hipStream_t s, int32_t *d_info);
+
+}  // namespace lmx

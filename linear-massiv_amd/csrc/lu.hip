@@ -1,0 +1,670 @@
+// LU factorisation with partial pivoting (right-looking, blocked) and the
+// LU solve, for row-major matrices.
+//
+// Panel kernel (getf2_panel_kernel): the m x kb panel (kb <= 32) is spread
+// over P cooperative workgroups of 256 threads, each owning 256*RPT
+// consecutive rows held in registers.  Per column j:
+//   1. each workgroup finds its candidate max |a_ij| over rows i >= j (ties:
+//      lowest row, NaN never chosen -- the rule of rawPivotSearch,
+//      src/.../Internal/Kernel.hs:1541-1556), wave shuffles + LDS;
+//   2. it publishes the candidate (value, row, full row data) in slot j%2
+//      and the owner of row j publishes row j; one grid barrier;
+//   3. every workgroup reduces the P candidates to the pivot p, the owners
+//      of rows j and p exchange them through the slots, and all rows i > j
+//      get l = a_ij / a_pj (a division, as the reference) and the rank-1
+//      update of the remaining panel columns.
+// The current column always sits in register 0: after each column the
+// row's registers rotate left by one, so the column loop needs no dynamic
+// register indexing.  Each row carries its panel-relative origin (a tag);
+// at the end the tags give the row-move map of the panel, and the rest of
+// the matrix (and the permutation vector) is permuted with one parallel
+// gather instead of a sequential swap loop.
+//
+// Driver (getrf_impl): outer panels of NB = 128 columns, each factored as
+// four 32-column sub-panels (panel kernel, gather, TRSM + GEMM update
+// inside the outer panel), then U12 = L11^{-1} A12 (blocked TRSM) and the
+// trailing update A22 -= L21 U12 on the MFMA GEMM.
+#include <climits>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace lmx {
+
+namespace {
+
+constexpr int KB = 32;        // panel kernel width
+constexpr int RPT = 2;        // rows per thread in the panel kernel
+constexpr int PANEL_ROWS = 256 * RPT;
+constexpr int MAXP = 256;     // max cooperative workgroups
+constexpr int NB_OUTER = 128; // outer panel width (k of the trailing GEMM)
+constexpr long SPIN_LIMIT = 1L << 22;
+
+template <class T>
+struct PanelSync {
+    unsigned counter;
+    unsigned error;
+    unsigned pad[14];
+    T cand_val[2][MAXP];
+    int cand_row[2][MAXP];
+    int cand_tag[2][MAXP];
+    T cand_data[2][MAXP][KB];
+    T diag_data[2][KB];
+    int diag_tag[2];
+};
+
+template <class T>
+__device__ inline void st(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline T ld(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// (a, row) beats (b, brow): larger magnitude, ties to the lower row; NaN never wins.
+template <class T>
+__device__ inline bool better(T a, int row, T b, int brow) {
+    return a > b || (a == b && row < brow);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
+                                                          int k0, int32_t *__restrict__ ipiv,
+                                                          int32_t *__restrict__ srcmap,
+                                                          int32_t *__restrict__ moves,
+                                                          int32_t *__restrict__ info,
+                                                          PanelSync<T> *__restrict__ sy, int P,
+                                                          unsigned base) {
+    const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    __shared__ T s_prow[KB], s_drow[KB];
+    __shared__ int s_ptag, s_dtag, s_p, s_pw;
+    __shared__ T s_wbest[4];
+    __shared__ int s_wrow[4];
+    __shared__ int s_pivs[KB];
+
+    T v[RPT][KB];
+    int tag[RPT], row[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        row[q] = w * PANEL_ROWS + q * 256 + t;
+        tag[q] = row[q];
+#pragma unroll
+        for (int c = 0; c < KB; ++c)
+            v[q][c] = (row[q] < m && c < kb) ? A[(int64_t)row[q] * lda + c] : T(0);
+    }
+    int local_info = 0;
+
+    for (int j = 0; j < kb; ++j) {
+        const int par = j & 1;
+        // 1. candidate of this workgroup
+        T best = T(-1);
+        int brow = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (row[q] >= j && row[q] < m) {
+                T a = fabs(v[q][0]);
+                if (better(a, row[q], best, brow)) {
+                    best = a;
+                    brow = row[q];
+                }
+            }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            T ob = __shfl_xor(best, off);
+            int orow = __shfl_xor(brow, off);
+            if (better(ob, orow, best, brow)) {
+                best = ob;
+                brow = orow;
+            }
+        }
+        if (lane == 0) {
+            s_wbest[wave] = best;
+            s_wrow[wave] = brow;
+        }
+        __syncthreads();
+        best = s_wbest[0];
+        brow = s_wrow[0];
+#pragma unroll
+        for (int x = 1; x < 4; ++x)
+            if (better(s_wbest[x], s_wrow[x], best, brow)) {
+                best = s_wbest[x];
+                brow = s_wrow[x];
+            }
+
+        int p;
+        if (P > 1) {
+            // 2. publish and synchronise
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (row[q] == brow) {
+#pragma unroll
+                    for (int c = 0; c < KB; ++c) st(&sy->cand_data[par][w][c], v[q][c]);
+                    st(&sy->cand_tag[par][w], tag[q]);
+                }
+                if (row[q] == j) {
+#pragma unroll
+                    for (int c = 0; c < KB; ++c) st(&sy->diag_data[par][c], v[q][c]);
+                    st(&sy->diag_tag[par], tag[q]);
+                }
+            }
+            if (t == 0) {
+                st(&sy->cand_val[par][w], best);
+                st(&sy->cand_row[par][w], brow);
+            }
+            __threadfence();
+            __syncthreads();
+            if (t == 0) {
+                __hip_atomic_fetch_add(&sy->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned target = base + (unsigned)(j + 1) * (unsigned)P;
+                long spins = 0;
+                while (__hip_atomic_load(&sy->counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    if (ld(&sy->error) != 0u) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > SPIN_LIMIT) {
+                        st(&sy->error, 1u);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            // 3. global pivot
+            if (wave == 0) {
+                T b = T(-1);
+                int r = INT_MAX, ws = -1;
+                for (int x = lane; x < P; x += 64) {
+                    T cb = ld(&sy->cand_val[par][x]);
+                    int cr = ld(&sy->cand_row[par][x]);
+                    if (better(cb, cr, b, r)) {
+                        b = cb;
+                        r = cr;
+                        ws = x;
+                    }
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    T ob = __shfl_xor(b, off);
+                    int orow = __shfl_xor(r, off);
+                    int ow = __shfl_xor(ws, off);
+                    if (better(ob, orow, b, r)) {
+                        b = ob;
+                        r = orow;
+                        ws = ow;
+                    }
+                }
+                if (lane == 0) {
+                    s_p = (r == INT_MAX) ? j : r;
+                    s_pw = (r == INT_MAX) ? -1 : ws;
+                }
+            }
+            __syncthreads();
+            p = s_p;
+            const int pw = s_pw;
+            if (t < KB) {
+                s_prow[t] = (pw >= 0) ? ld(&sy->cand_data[par][pw][t]) : ld(&sy->diag_data[par][t]);
+                s_drow[t] = ld(&sy->diag_data[par][t]);
+            }
+            if (t == 0) {
+                s_ptag = (pw >= 0) ? ld(&sy->cand_tag[par][pw]) : ld(&sy->diag_tag[par]);
+                s_dtag = ld(&sy->diag_tag[par]);
+            }
+            __syncthreads();
+        } else {
+            p = (brow == INT_MAX) ? j : brow;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (row[q] == p) {
+#pragma unroll
+                    for (int c = 0; c < KB; ++c) s_prow[c] = v[q][c];
+                    s_ptag = tag[q];
+                }
+                if (row[q] == j) {
+#pragma unroll
+                    for (int c = 0; c < KB; ++c) s_drow[c] = v[q][c];
+                    s_dtag = tag[q];
+                }
+            }
+            __syncthreads();
+        }
+
+        // 4. interchange rows j and p
+        if (p != j) {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (row[q] == p) {
+#pragma unroll
+                    for (int c = 0; c < KB; ++c) v[q][c] = s_drow[c];
+                    tag[q] = s_dtag;
+                } else if (row[q] == j) {
+#pragma unroll
+                    for (int c = 0; c < KB; ++c) v[q][c] = s_prow[c];
+                    tag[q] = s_ptag;
+                }
+            }
+        }
+        if (t == 0) s_pivs[j] = p;
+        if (w == 0 && t == 0) ipiv[k0 + j] = k0 + p;
+
+        // 5. multipliers and rank-1 update of the remaining panel columns
+        const T piv = s_prow[0];
+        if (piv != T(0)) {
+            const int lim = kb - j;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q)
+                if (row[q] > j && row[q] < m) {
+                    T l = v[q][0] / piv;
+                    v[q][0] = l;
+#pragma unroll
+                    for (int c = 1; c < KB; ++c)
+                        if (c < lim) v[q][c] = fma(-l, s_prow[c], v[q][c]);
+                }
+        } else if (local_info == 0) {
+            local_info = k0 + j + 1;
+        }
+        // 6. rotate: the next column moves to register 0
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            T f = v[q][0];
+#pragma unroll
+            for (int c = 0; c < KB - 1; ++c) v[q][c] = v[q][c + 1];
+            v[q][KB - 1] = f;
+        }
+        __syncthreads();
+    }
+
+    // After kb rotations register s holds column (s + kb) % KB.
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+        if (row[q] < m) {
+#pragma unroll
+            for (int s = 0; s < KB; ++s) {
+                int c = (s + kb) % KB;
+                if (c < kb) A[(int64_t)row[q] * lda + c] = v[q][s];
+            }
+            srcmap[row[q]] = tag[q];
+        }
+    if (w == 0 && t == 0) {
+        int cnt = 0;
+        for (int d = 0; d < kb && d < m; ++d) moves[1 + cnt++] = d;
+        for (int j = 0; j < kb; ++j) {
+            int pj = s_pivs[j];
+            if (pj < kb) continue;
+            bool dup = false;
+            for (int x = 0; x < j; ++x) dup |= (s_pivs[x] == pj);
+            if (!dup) moves[1 + cnt++] = pj;
+        }
+        moves[0] = cnt;
+        if (local_info != 0 && *info == 0) *info = local_info;
+    }
+}
+
+// X[d] <- X[src[d]] for every d of the move list, for columns [c0, c1) and
+// [c2, c3) of X (rows relative to X).  All reads of a column finish before
+// any write (one workgroup per 64 columns, barrier in between).
+template <class T>
+__global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
+                                                         int c2, int c3, const int32_t *__restrict__ moves,
+                                                         const int32_t *__restrict__ srcmap) {
+    constexpr int PER = (2 * KB + 3) / 4;  // moves per thread
+    const int t = threadIdx.x, cl = t & 63, grp = t >> 6;
+    const int n1 = c1 - c0;
+    const int cc = blockIdx.x * 64 + cl;
+    int col = -1;
+    if (cc < n1)
+        col = c0 + cc;
+    else if (cc - n1 < c3 - c2)
+        col = c2 + (cc - n1);
+    const int cnt = moves[0];
+    T val[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        int e = grp * PER + i;
+        val[i] = T(0);
+        if (col >= 0 && e < cnt) val[i] = X[(int64_t)srcmap[moves[1 + e]] * ldx + col];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        int e = grp * PER + i;
+        if (col >= 0 && e < cnt) X[(int64_t)moves[1 + e] * ldx + col] = val[i];
+    }
+}
+
+// perm[d] <- perm[src[d]] over the move list (one workgroup).
+__global__ __launch_bounds__(64) void perm_gather_kernel(int32_t *__restrict__ perm,
+                                                         const int32_t *__restrict__ moves,
+                                                         const int32_t *__restrict__ srcmap) {
+    const int t = threadIdx.x, cnt = moves[0];
+    int a = 0, b = 0;
+    if (t < cnt) a = perm[srcmap[moves[1 + t]]];
+    if (t + 64 < cnt) b = perm[srcmap[moves[1 + t + 64]]];
+    __syncthreads();
+    if (t < cnt) perm[moves[1 + t]] = a;
+    if (t + 64 < cnt) perm[moves[1 + t + 64]] = b;
+}
+
+__global__ void iota_kernel(int32_t *p, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void zero_u32_kernel(unsigned *p, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void finish_kernel(const unsigned *err, int32_t *info) {
+    if (threadIdx.x == 0 && *err != 0u) *info = -1;
+}
+
+// Swap list -> permutation vector: perm[i] = original row at position i.
+// One thread, the vector in LDS (n <= 16384).
+__global__ void ipiv_to_perm_kernel(const int32_t *__restrict__ ipiv, int32_t *__restrict__ perm, int n) {
+    extern __shared__ int32_t sp[];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sp[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < n; ++k) {
+            int p = ipiv[k];
+            if (p != k && p >= 0 && p < n) {
+                int a = sp[k];
+                sp[k] = sp[p];
+                sp[p] = a;
+            }
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
+}
+
+// out[i] = in[perm[i]] for an n x ncols block; then copied back by the caller.
+template <class T>
+__global__ void row_permute_kernel(const T *__restrict__ in, int64_t ldi, T *__restrict__ out, int64_t ldo,
+                                   const int32_t *__restrict__ perm, int n, int ncols) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * ncols) return;
+    int i = (int)(e / ncols), c = (int)(e % ncols);
+    out[(int64_t)i * ldo + c] = in[(int64_t)perm[i] * ldi + c];
+}
+
+template <class T>
+__global__ void copy2d_kernel(const T *__restrict__ in, int64_t ldi, T *__restrict__ out, int64_t ldo, int n,
+                              int ncols) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * ncols) return;
+    int i = (int)(e / ncols), c = (int)(e % ncols);
+    out[(int64_t)i * ldo + c] = in[(int64_t)i * ldi + c];
+}
+
+__global__ void set_i32_kernel(int32_t *p, int32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+template <class T>
+void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32_t *ipiv, int32_t *srcmap,
+                  int32_t *moves, int32_t *info, PanelSync<T> *sy, unsigned &base) {
+    int P = (int)cdiv(m, PANEL_ROWS);
+    if (P > MAXP) fail_arg("getrf: matrix too tall for the panel kernel");
+    TimedRegion tr("getrf_panel", s);
+    if (P == 1) {
+        hipLaunchKernelGGL(getf2_panel_kernel<T>, dim3(1), dim3(256), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
+                           moves, info, sy, 1, base);
+        LMX_LAUNCH_CHECK();
+        return;
+    }
+    void *args[] = {&m, &kb, &A, &lda, &k0, &ipiv, &srcmap, &moves, &info, &sy, &P, &base};
+    LMX_HIP(hipLaunchCooperativeKernel((const void *)getf2_panel_kernel<T>, dim3(P), dim3(256), args, 0, s));
+    base += (unsigned)P * (unsigned)kb;
+}
+
+template <class T>
+void gather_rows(hipStream_t s, T *X, int64_t ldx, int c0, int c1, int c2, int c3, const int32_t *moves,
+                 const int32_t *srcmap) {
+    int ncols = (c1 - c0) + (c3 - c2);
+    if (ncols <= 0) return;
+    TimedRegion tr("laswp", s);
+    hipLaunchKernelGGL(row_gather_kernel<T>, dim3((unsigned)cdiv(ncols, 64)), dim3(256), 0, s, X, ldx, c0, c1,
+                       c2, c3, moves, srcmap);
+    LMX_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void clear_info(hipStream_t s, int32_t *d_info) {
+    hipLaunchKernelGGL(set_i32_kernel, dim3(1), dim3(64), 0, s, d_info, 0);
+    LMX_LAUNCH_CHECK();
+}
+
+// getrf with optional permutation-vector output (d_perm may be null).
+template <class T>
+void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
+                int32_t *d_info) {
+    if (n < 0 || lda < std::max<int64_t>(1, n)) fail_arg("getrf: bad dimension or stride");
+    if (n > INT32_MAX / 2) fail_arg("getrf: n too large");
+    clear_info(s, d_info);
+    if (n == 0) return;
+    const int N = (int)n;
+    size_t sync_bytes = (sizeof(PanelSync<T>) + 255) & ~size_t(255);
+    Scratch scr(sync_bytes + sizeof(int32_t) * ((size_t)N + 2 * KB + 8) + 256, s);
+    auto *sy = scr.as<PanelSync<T>>();
+    int32_t *srcmap = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scr.p) + sync_bytes);
+    int32_t *moves = srcmap + N;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, s, (unsigned *)sy, 16);
+    LMX_LAUNCH_CHECK();
+    if (d_perm) {
+        hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, s, d_perm, N);
+        LMX_LAUNCH_CHECK();
+    }
+    unsigned base = 0;
+    for (int k0 = 0; k0 < N; k0 += NB_OUTER) {
+        const int nbw = std::min(NB_OUTER, N - k0);
+        for (int s0 = k0; s0 < k0 + nbw; s0 += KB) {
+            const int sb = std::min(KB, k0 + nbw - s0);
+            const int m = N - s0;
+            T *P0 = A + (int64_t)s0 * lda + s0;
+            launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
+            gather_rows<T>(s, A + (int64_t)s0 * lda, lda, 0, s0, s0 + sb, N, moves, srcmap);
+            if (d_perm) {
+                hipLaunchKernelGGL(perm_gather_kernel, dim3(1), dim3(64), 0, s, d_perm + s0, moves, srcmap);
+                LMX_LAUNCH_CHECK();
+            }
+            const int rest = k0 + nbw - (s0 + sb);
+            if (rest > 0) {
+                T *B12 = A + (int64_t)s0 * lda + s0 + sb;
+                trsm_left<T>(s, true, false, true, sb, rest, P0, lda, B12, lda);
+                gemm(s, false, false, m - sb, rest, sb, T(-1), P0 + (int64_t)sb * lda, lda, B12, lda, T(1),
+                     B12 + (int64_t)sb * lda, lda);
+            }
+        }
+        const int trail = N - (k0 + nbw);
+        if (trail > 0) {
+            T *A11 = A + (int64_t)k0 * lda + k0;
+            T *A12 = A11 + nbw;
+            trsm_left<T>(s, true, false, true, nbw, trail, A11, lda, A12, lda);
+            gemm(s, false, false, trail, trail, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12, lda, T(1),
+                 A12 + (int64_t)nbw * lda, lda);
+        }
+    }
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, s, &sy->error, d_info);
+    LMX_LAUNCH_CHECK();
+}
+
+template <class T>
+void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_info) {
+    getrf_impl<T>(s, n, A, lda, d_ipiv, nullptr, d_info);
+}
+
+// Solve with a permutation vector: B <- U^{-1} L^{-1} P B.
+template <class T>
+void getrs_perm(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda, const int32_t *d_perm, T *B,
+                int64_t ldb) {
+    if (n == 0 || nrhs == 0) return;
+    TimedRegion tr("getrs", s);
+    Scratch tmp((size_t)(n * nrhs) * sizeof(T), s);
+    unsigned g = (unsigned)cdiv(n * nrhs, 256);
+    hipLaunchKernelGGL(row_permute_kernel<T>, dim3(g), dim3(256), 0, s, B, ldb, tmp.as<T>(), nrhs, d_perm,
+                       (int)n, (int)nrhs);
+    LMX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(copy2d_kernel<T>, dim3(g), dim3(256), 0, s, tmp.as<T>(), nrhs, B, ldb, (int)n,
+                       (int)nrhs);
+    LMX_LAUNCH_CHECK();
+    trsm_left<T>(s, true, false, true, n, nrhs, LU, lda, B, ldb);
+    trsm_left<T>(s, false, false, false, n, nrhs, LU, lda, B, ldb);
+}
+
+template <class T>
+void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda, const int32_t *d_ipiv, T *B,
+           int64_t ldb) {
+    if (n == 0 || nrhs == 0) return;
+    if (n > 16384) fail_arg("getrs_dev: swap-list conversion supports n <= 16384 (use gesv_dev)");
+    Scratch perm((size_t)n * sizeof(int32_t), s);
+    hipLaunchKernelGGL(ipiv_to_perm_kernel, dim3(1), dim3(256), (size_t)n * sizeof(int32_t), s, d_ipiv,
+                       perm.as<int32_t>(), (int)n);
+    LMX_LAUNCH_CHECK();
+    getrs_perm<T>(s, n, nrhs, LU, lda, perm.as<int32_t>(), B, ldb);
+}
+
+template void getrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *);
+template void getrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *);
+template void getrf_impl<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *, int32_t *);
+template void getrf_impl<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *, int32_t *);
+template void getrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const int32_t *, double *,
+                            int64_t);
+template void getrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const int32_t *, float *,
+                           int64_t);
+template void getrs_perm<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const int32_t *,
+                                 double *, int64_t);
+template void getrs_perm<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const int32_t *, float *,
+                                int64_t);
+
+}  // namespace lmx
+
+using namespace lmx;
+
+namespace {
+
+void check_square(int64_t n, int64_t lda, const char *who) {
+    if (n < 0) fail_arg(std::string(who) + ": negative dimension");
+    if (lda < std::max<int64_t>(1, n)) fail_arg(std::string(who) + ": lda too small");
+}
+
+void check_rhs(int64_t nrhs, int64_t ldb, const char *who) {
+    if (nrhs < 0) fail_arg(std::string(who) + ": negative nrhs");
+    if (ldb < std::max<int64_t>(1, nrhs)) fail_arg(std::string(who) + ": ldb too small");
+}
+
+size_t extent(int64_t rows, int64_t cols, int64_t ld) {
+    return (rows > 0 && cols > 0) ? (size_t)((rows - 1) * ld + cols) * sizeof(double) : 0;
+}
+
+int read_info(hipStream_t s, const int32_t *d_info) {
+    int32_t info = 0;
+    LMX_HIP(hipMemcpyAsync(&info, d_info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LMX_HIP(hipStreamSynchronize(s));
+    if (info < 0) throw Error(LMX_ERR_INTERNAL, "panel synchronisation timed out");
+    return info;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmx_dgetrf_dev(void *stream, int64_t n, double *dA, int64_t lda, int32_t *d_ipiv, int32_t *d_info) {
+    return guarded([&] {
+        check_square(n, lda, "getrf");
+        getrf<double>(resolve_stream(stream), n, dA, lda, d_ipiv, d_info);
+        return LMX_OK;
+    });
+}
+
+int lmx_dgetrs_dev(void *stream, int64_t n, int64_t nrhs, const double *dLU, int64_t lda, const int32_t *d_ipiv,
+                   double *dB, int64_t ldb) {
+    return guarded([&] {
+        check_square(n, lda, "getrs");
+        check_rhs(nrhs, ldb, "getrs");
+        getrs<double>(resolve_stream(stream), n, nrhs, dLU, lda, d_ipiv, dB, ldb);
+        return LMX_OK;
+    });
+}
+
+int lmx_dgesv_dev(void *stream, int64_t n, int64_t nrhs, double *dA, int64_t lda, int32_t *d_ipiv, double *dB,
+                  int64_t ldb, int32_t *d_info) {
+    return guarded([&] {
+        check_square(n, lda, "gesv");
+        check_rhs(nrhs, ldb, "gesv");
+        hipStream_t s = resolve_stream(stream);
+        Scratch perm((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t), s);
+        getrf_impl<double>(s, n, dA, lda, d_ipiv, perm.as<int32_t>(), d_info);
+        getrs_perm<double>(s, n, nrhs, dA, lda, perm.as<int32_t>(), dB, ldb);
+        return LMX_OK;
+    });
+}
+
+int lmx_dgetrf(int64_t n, double *A, int64_t lda, int32_t *ipiv) {
+    return guarded([&] {
+        check_square(n, lda, "getrf");
+        if (n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda);
+        double *dA = (double *)staging(c, 0, ab);
+        int32_t *dp = (int32_t *)staging(c, 1, (size_t)n * 4 + 64);
+        int32_t *di = dp + n;
+        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        getrf<double>(c.stream, n, dA, lda, dp, di);
+        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipMemcpyAsync(ipiv, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+        return read_info(c.stream, di);
+    });
+}
+
+int lmx_dgetrs(int64_t n, int64_t nrhs, const double *LU, int64_t lda, const int32_t *ipiv, double *B,
+               int64_t ldb) {
+    return guarded([&] {
+        check_square(n, lda, "getrs");
+        check_rhs(nrhs, ldb, "getrs");
+        if (n == 0 || nrhs == 0) return LMX_OK;
+        for (int64_t k = 0; k < n; ++k)
+            if (ipiv[k] < k || ipiv[k] >= n) fail_arg("getrs: ipiv entry out of range");
+        // swap list -> permutation on the host: O(n)
+        std::vector<int32_t> perm((size_t)n);
+        for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (int32_t)i;
+        for (int64_t k = 0; k < n; ++k) std::swap(perm[(size_t)k], perm[(size_t)ipiv[k]]);
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda), bb = extent(n, nrhs, ldb);
+        double *dA = (double *)staging(c, 0, ab), *dB = (double *)staging(c, 1, bb);
+        int32_t *dp = (int32_t *)staging(c, 2, (size_t)n * 4);
+        LMX_HIP(hipMemcpyAsync(dA, LU, ab, hipMemcpyHostToDevice, c.stream));
+        LMX_HIP(hipMemcpyAsync(dB, B, bb, hipMemcpyHostToDevice, c.stream));
+        LMX_HIP(hipMemcpyAsync(dp, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, c.stream));
+        getrs_perm<double>(c.stream, n, nrhs, dA, lda, dp, dB, ldb);
+        LMX_HIP(hipMemcpyAsync(B, dB, bb, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipStreamSynchronize(c.stream));
+        return LMX_OK;
+    });
+}
+
+int lmx_dgesv(int64_t n, int64_t nrhs, double *A, int64_t lda, int32_t *ipiv, double *B, int64_t ldb) {
+    return guarded([&] {
+        check_square(n, lda, "gesv");
+        check_rhs(nrhs, ldb, "gesv");
+        if (n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda), bb = extent(n, nrhs, ldb);
+        double *dA = (double *)staging(c, 0, ab), *dB = (double *)staging(c, 1, bb);
+        int32_t *dp = (int32_t *)staging(c, 2, (size_t)n * 8 + 64);
+        int32_t *dperm = dp + n, *di = dperm + n;
+        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        if (bb) LMX_HIP(hipMemcpyAsync(dB, B, bb, hipMemcpyHostToDevice, c.stream));
+        getrf_impl<double>(c.stream, n, dA, lda, dp, dperm, di);
+        getrs_perm<double>(c.stream, n, nrhs, dA, lda, dperm, dB, ldb);
+        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
+        if (bb) LMX_HIP(hipMemcpyAsync(B, dB, bb, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipMemcpyAsync(ipiv, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+        return read_info(c.stream, di);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+// Mixed-precision solves with FP64 iterative refinement.
+//
+// A (FP64, unchanged) is rounded to the factor precision and factored there
+// (LU with partial pivoting, or Cholesky) on the MFMA kernels.  Then, in
+// FP64 on the device:
+//   X_0 = 0, R_0 = B
+//   repeat: D = solve_lowprec(R);  X += D;  R = B - A X  (FP64 MFMA GEMM);
+//           eta = max_c ||r_c||_inf / (||A||_inf ||x_c||_inf + ||b_c||_inf)
+// until every column meets LAPACK dsgesv's test
+//   ||r_c||_inf <= sqrt(n) eps ||A||_inf ||x_c||_inf,
+// or eta stops halving, or 30 steps.  A solve that does not converge is
+// redone with the FP64 factorisation (fell_back = 1).  Only eta (one double)
+// crosses to the host per step.
+#include <cmath>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace lmx {
+
+namespace {
+
+constexpr int ITMAX = 30;
+constexpr double EPS64 = 2.220446049250313e-16;
+
+__global__ void d2f_kernel(const double *__restrict__ in, int64_t ldi, float *__restrict__ out, int64_t ldo,
+                           int64_t rows, int64_t cols) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * cols) return;
+    int64_t i = e / cols, c = e % cols;
+    out[i * ldo + c] = (float)in[i * ldi + c];
+}
+
+// X += (double) D ; optionally X = D when `assign`.
+__global__ void f2d_acc_kernel(const float *__restrict__ D, int64_t ldd, double *__restrict__ X, int64_t ldx,
+                               int64_t rows, int64_t cols, int assign) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * cols) return;
+    int64_t i = e / cols, c = e % cols;
+    double d = (double)D[i * ldd + c];
+    X[i * ldx + c] = assign ? d : X[i * ldx + c] + d;
+}
+
+__global__ void copy_d_kernel(const double *__restrict__ in, int64_t ldi, double *__restrict__ out, int64_t ldo,
+                              int64_t rows, int64_t cols) {
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * cols) return;
+    int64_t i = e / cols, c = e % cols;
+    out[i * ldo + c] = in[i * ldi + c];
+}
+
+__global__ void zero_d_kernel(double *p, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Non-negative doubles order like their bit patterns: max via integer atomics.
+__device__ inline void atomic_max_nonneg(double *addr, double v) {
+    if (!(v >= 0.0)) v = __builtin_huge_val();  // NaN poisons the norm
+    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// ||A||_inf = max_i sum_j |a_ij| : one workgroup per row block of 64 rows.
+__global__ __launch_bounds__(256) void norm_inf_rows_kernel(const double *__restrict__ A, int64_t lda,
+                                                            int64_t n, double *__restrict__ out) {
+    __shared__ double part[4][64];
+    const int t = threadIdx.x, r = t & 63, q = t >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + r;
+    double s = 0.0;
+    if (i < n)
+        for (int64_t j = q; j < n; j += 4) s += fabs(A[i * lda + j]);
+    part[q][r] = s;
+    __syncthreads();
+    if (q == 0 && i < n) atomic_max_nonneg(out, part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+}
+
+// out[c] = max_i |X[i][c]| for an n x ncols block.
+__global__ __launch_bounds__(256) void colmax_kernel(const double *__restrict__ X, int64_t ldx, int64_t n,
+                                                     int ncols, double *__restrict__ out) {
+    const int64_t e0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t total = n * ncols;
+    for (int64_t e = e0; e < total; e += (int64_t)gridDim.x * 256) {
+        int64_t i = e / ncols;
+        int c = (int)(e % ncols);
+        atomic_max_nonneg(&out[c], fabs(X[i * ldx + c]));
+    }
+}
+
+// norms layout: [0] ||A||, [1..1+k) ||b_c||, [1+k..1+2k) ||x_c||, [1+2k..1+3k) ||r_c||,
+// [1+3k] eta, [2+3k] all-columns-pass flag (1.0 / 0.0)
+__global__ void eta_kernel(double *norms, int k, double crit) {
+    if (threadIdx.x != 0) return;
+    double nA = norms[0], eta = 0.0, pass = 1.0;
+    for (int c = 0; c < k; ++c) {
+        double nb = norms[1 + c], nx = norms[1 + k + c], nr = norms[1 + 2 * k + c];
+        double den = nA * nx + nb;
+        double e = den > 0.0 ? nr / den : (nr > 0.0 ? __builtin_huge_val() : 0.0);
+        if (!(e <= eta)) eta = e;  // keeps NaN
+        if (!(nr <= crit * nA * nx)) pass = 0.0;
+    }
+    norms[1 + 3 * k] = eta;
+    norms[2 + 3 * k] = pass;
+}
+
+unsigned grid_for(int64_t elems) { return (unsigned)std::min<int64_t>(cdiv(elems, 256), 1 << 20); }
+
+template <class F>
+void launch1d(hipStream_t s, int64_t elems, F launch) {
+    if (elems > 0) launch(dim3((unsigned)cdiv(elems, 256)));
+}
+
+struct RefineOut {
+    int iterations = 0;
+    bool converged = false;
+    bool fell_back = false;
+    int info = 0;
+    double eta = 0.0;
+};
+
+// precision: LMX_PREC_F32 (factor in FP32).
+RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_t k, const double *A,
+                       int64_t lda, const double *B, int64_t ldb, double *X, int64_t ldx) {
+    RefineOut out;
+    if (precision != LMX_PREC_F32) fail_arg("mixed solve: factor precision must be LMX_PREC_F32");
+    if (n == 0 || k == 0) {
+        out.converged = true;
+        return out;
+    }
+    TimedRegion trall("refine", s);
+    const size_t nn = (size_t)n * (size_t)n, nk = (size_t)n * (size_t)k;
+    Scratch sA32(nn * sizeof(float), s), sR(nk * sizeof(double), s), sR32(nk * sizeof(float), s),
+        sI((size_t)(2 * n + 8) * sizeof(int32_t), s), sN((size_t)(3 * k + 8) * sizeof(double), s);
+    float *A32 = sA32.as<float>(), *R32 = sR32.as<float>();
+    double *R = sR.as<double>(), *norms = sN.as<double>();
+    int32_t *ipiv = sI.as<int32_t>(), *perm = ipiv + n, *dinfo = perm + n;
+    const int K = (int)k;
+
+    launch1d(s, (int64_t)nn, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, A, lda, A32, n, n, n); });
+    LMX_LAUNCH_CHECK();
+    if (spd)
+        potrf<float>(s, n, A32, n, dinfo);
+    else
+        getrf_impl<float>(s, n, A32, n, ipiv, perm, dinfo);
+    int32_t hinfo = 0;
+    LMX_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LMX_HIP(hipStreamSynchronize(s));
+    out.info = hinfo;
+
+    const double crit = std::sqrt((double)n) * EPS64;
+    bool lowprec_ok = (hinfo == 0);
+    if (lowprec_ok) {
+        hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
+        LMX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, A, lda, n, norms);
+        LMX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, B, ldb, n, K, norms + 1);
+        LMX_LAUNCH_CHECK();
+        // R = B
+        launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
+        double prev = INFINITY;
+        for (int it = 0; it < ITMAX; ++it) {
+            launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, R, k, R32, k, n, k); });
+            if (spd)
+                potrs<float>(s, n, k, A32, n, R32, k);
+            else
+                getrs_perm<float>(s, n, k, A32, n, perm, R32, k);
+            launch1d(s, (int64_t)nk, [&](dim3 g) {
+                hipLaunchKernelGGL(f2d_acc_kernel, g, dim3(256), 0, s, R32, k, X, ldx, n, k, it == 0 ? 1 : 0);
+            });
+            // R = B - A X
+            launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
+            gemm(s, false, false, n, k, n, -1.0, A, lda, X, ldx, 1.0, R, k);
+            hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(2 * K + 2, 256)), dim3(256), 0, s, norms + 1 + K, 2 * K + 2);
+            hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, X, ldx, n, K, norms + 1 + K);
+            hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, R, k, n, K, norms + 1 + 2 * K);
+            hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
+            LMX_LAUNCH_CHECK();
+            double h[2];
+            LMX_HIP(hipMemcpyAsync(h, norms + 1 + 3 * K, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            LMX_HIP(hipStreamSynchronize(s));
+            out.iterations = it + 1;
+            out.eta = h[0];
+            if (h[1] == 1.0) {
+                out.converged = true;
+                break;
+            }
+            if (!(h[0] < 0.5 * prev)) break;  // stagnation (or NaN)
+            prev = h[0];
+        }
+    }
+    if (!out.converged) {
+        // FP64 factorisation of a copy of A.
+        out.fell_back = true;
+        Scratch sA64(nn * sizeof(double), s);
+        double *A64 = sA64.as<double>();
+        launch1d(s, (int64_t)nn, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, A, lda, A64, n, n, n); });
+        launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, X, ldx, n, k); });
+        if (spd) {
+            potrf<double>(s, n, A64, n, dinfo);
+            potrs<double>(s, n, k, A64, n, X, ldx);
+        } else {
+            getrf_impl<double>(s, n, A64, n, ipiv, perm, dinfo);
+            getrs_perm<double>(s, n, k, A64, n, perm, X, ldx);
+        }
+        // report the backward error of the FP64 solve
+        launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
+        gemm(s, false, false, n, k, n, -1.0, A, lda, X, ldx, 1.0, R, k);
+        hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
+        hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, A, lda, n, norms);
+        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, B, ldb, n, K, norms + 1);
+        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, X, ldx, n, K, norms + 1 + K);
+        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, R, k, n, K, norms + 1 + 2 * K);
+        hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
+        LMX_LAUNCH_CHECK();
+        double h[2];
+        int32_t i64 = 0;
+        LMX_HIP(hipMemcpyAsync(h, norms + 1 + 3 * K, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        LMX_HIP(hipMemcpyAsync(&i64, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LMX_HIP(hipStreamSynchronize(s));
+        out.eta = h[0];
+        out.info = i64;
+        out.converged = (h[1] == 1.0);
+    }
+    return out;
+}
+
+void fill(lmx_refine_info *info, const RefineOut &o) {
+    if (!info) return;
+    info->iterations = o.iterations;
+    info->converged = o.converged ? 1 : 0;
+    info->fell_back = o.fell_back ? 1 : 0;
+    info->info = o.info;
+    info->backward_error = o.eta;
+}
+
+void check_args(int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldx) {
+    if (n < 0 || k < 0) fail_arg("mixed solve: negative dimension");
+    if (lda < std::max<int64_t>(1, n) || ldb < std::max<int64_t>(1, k) || ldx < std::max<int64_t>(1, k))
+        fail_arg("mixed solve: bad stride");
+}
+
+int host_mixed(bool spd, int precision, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,
+               int64_t ldb, double *X, int64_t ldx, lmx_refine_info *info) {
+    check_args(n, k, lda, ldb, ldx);
+    if (n == 0 || k == 0) {
+        fill(info, RefineOut{0, true, false, 0, 0.0});
+        return LMX_OK;
+    }
+    Context &c = ctx();
+    std::lock_guard<std::mutex> g(c.host_mu);
+    auto ext = [](int64_t r, int64_t cc, int64_t ld) { return (size_t)((r - 1) * ld + cc) * sizeof(double); };
+    size_t ab = ext(n, n, lda), bb = ext(n, k, ldb), xb = ext(n, k, ldx);
+    double *dA = (double *)staging(c, 0, ab), *dB = (double *)staging(c, 1, bb), *dX = (double *)staging(c, 2, xb);
+    LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+    LMX_HIP(hipMemcpyAsync(dB, B, bb, hipMemcpyHostToDevice, c.stream));
+    if (ldx > k) LMX_HIP(hipMemcpyAsync(dX, X, xb, hipMemcpyHostToDevice, c.stream));
+    RefineOut o = refine_solve(c.stream, spd, precision, n, k, dA, lda, dB, ldb, dX, ldx);
+    LMX_HIP(hipMemcpyAsync(X, dX, xb, hipMemcpyDeviceToHost, c.stream));
+    LMX_HIP(hipStreamSynchronize(c.stream));
+    fill(info, o);
+    if (o.info < 0) throw Error(LMX_ERR_INTERNAL, "panel synchronisation timed out");
+    return o.converged ? (o.fell_back ? o.info : LMX_OK) : LMX_ERR_NOT_CONVERGED;
+}
+
+}  // namespace
+
+}  // namespace lmx
+
+using namespace lmx;
+
+extern "C" {
+
+int lmx_dgesv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs, const double *dA, int64_t lda,
+                        const double *dB, int64_t ldb, double *dX, int64_t ldx, lmx_refine_info *info) {
+    return guarded([&] {
+        check_args(n, nrhs, lda, ldb, ldx);
+        RefineOut o = refine_solve(resolve_stream(stream), false, precision, n, nrhs, dA, lda, dB, ldb, dX, ldx);
+        fill(info, o);
+        return o.converged ? (o.fell_back ? o.info : LMX_OK) : LMX_ERR_NOT_CONVERGED;
+    });
+}
+
+int lmx_dposv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs, const double *dA, int64_t lda,
+                        const double *dB, int64_t ldb, double *dX, int64_t ldx, lmx_refine_info *info) {
+    return guarded([&] {
+        check_args(n, nrhs, lda, ldb, ldx);
+        RefineOut o = refine_solve(resolve_stream(stream), true, precision, n, nrhs, dA, lda, dB, ldb, dX, ldx);
+        fill(info, o);
+        return o.converged ? (o.fell_back ? o.info : LMX_OK) : LMX_ERR_NOT_CONVERGED;
+    });
+}
+
+int lmx_dgesv_mixed(int precision, int64_t n, int64_t nrhs, const double *A, int64_t lda, const double *B,
+                    int64_t ldb, double *X, int64_t ldx, lmx_refine_info *info) {
+    return guarded([&] { return host_mixed(false, precision, n, nrhs, A, lda, B, ldb, X, ldx, info); });
+}
+
+int lmx_dposv_mixed(int precision, int64_t n, int64_t nrhs, const double *A, int64_t lda, const double *B,
+                    int64_t ldb, double *X, int64_t ldx, lmx_refine_info *info) {
+    return guarded([&] { return host_mixed(true, precision, n, nrhs, A, lda, B, ldb, X, ldx, info); });
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// Context, error reporting, staging/scratch memory and kernel-family timing.
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace lmx {
+
+namespace {
+thread_local std::string g_last_error = "no error";
+constexpr int kMaxDevices = 64;
+Context *g_ctx[kMaxDevices] = {};
+std::mutex g_ctx_mu;
+
+const char *kFamilies[] = {"gemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
+                           "syrk", "getrs", "refine"};
+constexpr int kNumFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
+struct EventPair {
+    hipEvent_t a, b;
+};
+std::mutex g_time_mu;
+bool g_timing = false;
+std::vector<EventPair> g_pairs[kNumFamilies];
+std::vector<hipEvent_t> g_event_pool;
+
+int family_slot(const char *f) {
+    for (int i = 0; i < kNumFamilies; ++i)
+        if (std::strcmp(f, kFamilies[i]) == 0) return i;
+    return -1;
+}
+
+hipEvent_t take_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    LMX_HIP(hipEventCreate(&e));
+    return e;
+}
+
+Context *make_context(int dev) {
+    auto *c = new Context();
+    c->device = dev;
+    LMX_HIP(hipSetDevice(dev));
+    // Default priority, non-blocking with respect to the legacy stream.
+    LMX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    LMX_HIP(hipGetDeviceProperties(&prop, dev));
+    c->cu_count = prop.multiProcessorCount;
+    return c;
+}
+}  // namespace
+
+void throw_hip(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+                  hipGetErrorString(e), what, file, line);
+    throw Error(e == hipErrorOutOfMemory ? LMX_ERR_NOMEM : LMX_ERR_DEVICE, buf);
+}
+
+void set_last_error(const std::string &m) { g_last_error = m; }
+
+Context &ctx() {
+    int dev = 0;
+    LMX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(LMX_ERR_DEVICE, "device id out of range");
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    if (!g_ctx[dev]) g_ctx[dev] = make_context(dev);
+    return *g_ctx[dev];
+}
+
+// NULL is HIP's default (legacy) stream, which orders against every other
+// blocking stream -- the stream torch uses unless told otherwise.
+hipStream_t resolve_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+void *staging(Context &c, int slot, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (c.staging_bytes[slot] < bytes) {
+        if (c.staging[slot]) {
+            LMX_HIP(hipStreamSynchronize(c.stream));
+            LMX_HIP(hipFree(c.staging[slot]));
+            c.staging[slot] = nullptr;
+            c.staging_bytes[slot] = 0;
+        }
+        size_t sz = bytes + bytes / 8;
+        LMX_HIP(hipMalloc(&c.staging[slot], sz));
+        c.staging_bytes[slot] = sz;
+    }
+    return c.staging[slot];
+}
+
+Scratch::Scratch(size_t bytes, hipStream_t st) : s(st) {
+    if (bytes == 0) bytes = 256;
+    LMX_HIP(hipMallocAsync(&p, bytes, s));
+}
+
+Scratch::~Scratch() {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
+bool timing_enabled() { return g_timing; }
+
+TimedRegion::TimedRegion(const char *family, hipStream_t st) : s(st) {
+    if (!g_timing) return;
+    std::lock_guard<std::mutex> g(g_time_mu);
+    slot = family_slot(family);
+    if (slot < 0) return;
+    e0 = take_event();
+    e1 = take_event();
+    LMX_HIP(hipEventRecord(e0, s));
+}
+
+TimedRegion::~TimedRegion() {
+    if (slot < 0) return;
+    std::lock_guard<std::mutex> g(g_time_mu);
+    if (hipEventRecord(e1, s) == hipSuccess) g_pairs[slot].push_back({e0, e1});
+}
+
+}  // namespace lmx
+
+using namespace lmx;
+
+extern "C" {
+
+int lmx_init(int device) {
+    return guarded([&] {
+        int n = 0;
+        LMX_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail_arg("lmx_init: no such device");
+        LMX_HIP(hipSetDevice(device));
+        (void)ctx();
+        return LMX_OK;
+    });
+}
+
+int lmx_finalize(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(g_ctx_mu);
+        int cur = 0;
+        LMX_HIP(hipGetDevice(&cur));
+        for (int d = 0; d < kMaxDevices; ++d) {
+            Context *c = g_ctx[d];
+            if (!c) continue;
+            LMX_HIP(hipSetDevice(d));
+            LMX_HIP(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < 4; ++i)
+                if (c->staging[i]) LMX_HIP(hipFree(c->staging[i]));
+            LMX_HIP(hipStreamDestroy(c->stream));
+            delete c;
+            g_ctx[d] = nullptr;
+        }
+        LMX_HIP(hipSetDevice(cur));
+        return LMX_OK;
+    });
+}
+
+const char *lmx_last_error(void) { return g_last_error.c_str(); }
+
+int lmx_version(void) { return 100; }
+
+int lmx_kernel_timing(int enable) {
+    std::lock_guard<std::mutex> g(g_time_mu);
+    g_timing = enable != 0;
+    return LMX_OK;
+}
+
+int lmx_kernel_timing_reset(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(g_time_mu);
+        for (auto &v : g_pairs)
+            for (auto &p : v) {
+                LMX_HIP(hipEventSynchronize(p.b));
+                g_event_pool.push_back(p.a);
+                g_event_pool.push_back(p.b);
+            }
+        for (auto &v : g_pairs) v.clear();
+        return LMX_OK;
+    });
+}
+
+int lmx_kernel_timing_get(const char *family, int64_t *count, double *total_ms) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> g(g_time_mu);
+        int s = family_slot(family ? family : "");
+        if (s < 0) fail_arg("unknown kernel family");
+        double tot = 0;
+        for (auto &p : g_pairs[s]) {
+            LMX_HIP(hipEventSynchronize(p.b));
+            float ms = 0;
+            LMX_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            tot += ms;
+        }
+        if (count) *count = (int64_t)g_pairs[s].size();
+        if (total_ms) *total_ms = tot;
+        return LMX_OK;
+    });
+}
+
+}  // extern "C"

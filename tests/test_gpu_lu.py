@@ -1,0 +1,169 @@
+"""GPU LU factor/solve through the C ABI against LAPACK golden factors, the
+oracle's luSolveP / lu, the reference KATs and the backward-error bound
+eta <= 10 n eps (scaledResidualLinear, test/Test/Residuals.hs:69-78)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+EPS = 2.220446049250313e-16
+
+
+def eta(orc, A, x, b):
+    return orc.scaled_residual_linear(A, x, b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 10, 12, 33, 63, 64, 65, 100, 130])
+def test_getrf_vs_lapack_golden(lmx_gpu, orc, golden, n):
+    A = golden[f"A_{n}"]
+    LU, ipiv, info = lmx_gpu.getrf(A)
+    assert info == 0
+    np.testing.assert_array_equal(ipiv, golden[f"lapack_ipiv_{n}"])
+    np.testing.assert_allclose(LU, golden[f"lapack_lu_{n}"], rtol=0, atol=1e-12 * max(1, n))
+    # same pivots as the oracle's luSolveP factorisation
+    LUo, swo = orc.lu_p_factor(A)
+    np.testing.assert_array_equal(ipiv, swo)
+
+
+@pytest.mark.parametrize("n", [200, 511, 512, 513, 1000])
+def test_getrf_vs_oracle(lmx_gpu, orc, n):
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (n, n))
+    LU, ipiv, info = lmx_gpu.getrf(A)
+    LUo, swo = orc.lu_p_factor(A)
+    assert info == 0
+    np.testing.assert_array_equal(ipiv, swo)
+    assert np.abs(LU - LUo).max() <= 1e-10
+    perm = lmx_gpu.swaps_to_perm(ipiv)
+    assert orc.scaled_residual_lu(A, LU, perm) < 10
+
+
+def test_lu_kat_3x3(lmx_gpu):
+    # test/Test/Solve.hs:107-119, closed form x = [1, -1, 0]
+    A = np.array([[2.0, 1, 1], [4, 3, 3], [8, 7, 9]])
+    x = lmx_gpu.luSolve(A, np.ones(3))
+    np.testing.assert_allclose(x, [1, -1, 0], atol=4 * EPS)
+    LU, perm = lmx_gpu.lu(A)
+    np.testing.assert_array_equal(perm, [2, 0, 1])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 32, 33, 64, 100, 127, 128, 129, 257, 500, 1000])
+@pytest.mark.parametrize("nrhs", [1, 16])
+def test_gesv_backward_error_and_oracle(lmx_gpu, orc, n, nrhs):
+    rng = np.random.default_rng(n * 31 + nrhs)
+    A = rng.uniform(-1, 1, (n, n))
+    B = rng.uniform(-1, 1, (n, nrhs))
+    X, LU, ipiv, info = lmx_gpu.gesv(A, B)
+    assert info == 0
+    assert eta(orc, A, X, B) <= 10 * n * EPS
+    # parity with the reference algorithm: the oracle's luSolveP, column by column
+    kappa = np.linalg.cond(A, np.inf)
+    for j in range(min(nrhs, 3)):
+        xo = orc.luSolveP(A, B[:, j])
+        err = np.abs(X[:, j] - xo).max() / max(np.abs(xo).max(), 1e-300)
+        assert err <= 10 * orc.condition_tolerance(kappa, n) + 1e-14
+
+
+@pytest.mark.parametrize("n", [3, 5, 10, 12])
+def test_gesv_vs_mpmath(lmx_gpu, orc, golden, n):
+    A, b, xe = golden[f"A_{n}"], golden[f"B_{n}"][:, 0], golden[f"mp_x_{n}"]
+    x = lmx_gpu.luSolveP(A, b)
+    kappa = np.linalg.cond(A, np.inf)
+    assert np.abs(x - xe).max() / np.abs(xe).max() <= 10 * orc.condition_tolerance(kappa, n)
+
+
+@pytest.mark.parametrize("n", [6, 70, 600])
+def test_permutation_matrix_exact(lmx_gpu, n):
+    rng = np.random.default_rng(n)
+    P = np.eye(n)[rng.permutation(n)]
+    b = rng.uniform(-1, 1, n)
+    np.testing.assert_array_equal(lmx_gpu.luSolveP(P, b), P.T @ b)
+    LU, perm = lmx_gpu.lu(P)
+    np.testing.assert_array_equal(LU, np.eye(n))
+    np.testing.assert_array_equal(P[perm], np.eye(n))
+
+
+def test_hilbert5(lmx_gpu, orc):
+    # test/Test/Solve.hs:157-165
+    n = 5
+    H = 1.0 / (np.arange(n)[:, None] + np.arange(n)[None, :] + 1)
+    b = H @ np.arange(1, n + 1, dtype=float)
+    x = lmx_gpu.luSolve(H, b)
+    assert eta(orc, H, x, b) < 1e-4
+    assert eta(orc, H, x, b) <= 10 * n * EPS
+
+
+def test_tie_breaking_first_max(lmx_gpu, orc):
+    # rawPivotSearch keeps the first of equal |a_ik| (Internal/Kernel.hs:1541-1556)
+    A = np.array([[1.0, 2.0], [-1.0, 3.0]])
+    LU, ipiv, info = lmx_gpu.getrf(A)
+    np.testing.assert_array_equal(ipiv, [0, 1])
+    np.testing.assert_array_equal(LU, orc.lu_p_factor(A)[0])
+    # ties across panel workgroups: equal magnitudes in rows 3 and 900
+    n = 1000
+    rng = np.random.default_rng(3)
+    A = rng.uniform(-0.5, 0.5, (n, n))
+    A[3, 0], A[900, 0] = -1.0, 1.0
+    LU, ipiv, info = lmx_gpu.getrf(A)
+    assert ipiv[0] == 3
+    np.testing.assert_array_equal(ipiv, orc.lu_p_factor(A)[1])
+
+
+@pytest.mark.parametrize("n,zero_col", [(5, 2), (100, 40), (700, 650)])
+def test_singular_info(lmx_gpu, n, zero_col):
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (n, n))
+    A[:, zero_col] = 0.0
+    LU, ipiv, info = lmx_gpu.getrf(A)
+    assert info == zero_col + 1
+
+
+def test_getrs_reuses_factors(lmx_gpu, orc):
+    rng = np.random.default_rng(11)
+    n = 300
+    A = rng.uniform(-1, 1, (n, n))
+    LU, ipiv, info = lmx_gpu.getrf(A)
+    for nrhs in (1, 5, 64):
+        B = rng.uniform(-1, 1, (n, nrhs))
+        X = lmx_gpu.getrs(LU, ipiv, B)
+        assert eta(orc, A, X, B) <= 10 * n * EPS
+
+
+def test_device_getrs_with_swap_list(lmx_gpu, orc):
+    import torch
+    rng = np.random.default_rng(12)
+    n, k = 700, 4
+    A = rng.uniform(-1, 1, (n, n))
+    B = rng.uniform(-1, 1, (n, k))
+    dA = torch.tensor(A, device="cuda")
+    dB = torch.tensor(B, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lmx_gpu.getrf_dev(dA, ipiv, info)
+    lmx_gpu.getrs_dev(dA, ipiv, dB)
+    torch.cuda.synchronize()
+    assert info.item() == 0
+    assert eta(orc, A, dB.cpu().numpy(), B) <= 10 * n * EPS
+
+
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_device_gesv_large(lmx_gpu, orc, n):
+    # the n = 8192, 16 right-hand-side workload: backward error, size-independent
+    import torch
+    k = 16
+    g = torch.Generator(device="cuda").manual_seed(n)
+    A = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    B = torch.rand(n, k, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    LU = A.clone()
+    X = B.clone()
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lmx_gpu.gesv_dev(LU, ipiv, X, info)
+    torch.cuda.synchronize()
+    assert info.item() == 0
+    R = B - A @ X
+    num = R.abs().max(dim=0).values
+    den = A.abs().sum(1).max() * X.abs().max(dim=0).values + B.abs().max(dim=0).values
+    assert (num / den).max().item() <= 10 * n * EPS
+    # pivots are a valid swap list
+    iv = ipiv.cpu().numpy()
+    assert np.all(iv >= np.arange(n)) and np.all(iv < n)
