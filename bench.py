@@ -1,0 +1,301 @@
+#!/usr/bin/env python3
+"""Benchmark of the dense-solve path on one MI355X.
+
+Default workload (the configuration the metric is quoted on): luSolve of
+A x = b, n = 16384, 1 right-hand side, A = U diag(sigma) V^T with Haar U, V
+and geometric sigma (kappa = 1e6), factored in FP32 on MFMA and refined to
+FP64 accuracy on the device.  One step = one FP64-accurate solve (factor +
+refinement) with A, b already resident in HBM.  The same inputs are also
+solved by the plain FP64 path (FP64 factor + solve) for the "vs plain FP64
+GPU" part of the metric.
+
+Prints ONE JSON line.  Other workloads (--workload) exist for profiling:
+lu_fp64 / chol_fp64 (n = 8192, 16 RHS), gemm_fp64 (8192^3), chol_mixed.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "linear-massiv_amd"))
+
+EPS = 2.220446049250313e-16
+PEAK = {"gemm": 78.6, "sgemm": 157.3, "syrk": 78.6}  # TFLOP/s, MI355X dense MFMA
+KERNEL_FAMILIES = ["gemm", "sgemm", "syrk", "getrf_panel", "trsm", "laswp", "potrf_diag"]
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--workload", default="lu_mixed",
+                   choices=["lu_mixed", "chol_mixed", "lu_fp64", "chol_fp64", "gemm_fp64"])
+    p.add_argument("--n", type=int, default=None)
+    p.add_argument("--nrhs", type=int, default=None)
+    p.add_argument("--kappa", type=float, default=1e6)
+    p.add_argument("--seed", type=int, default=2026)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-fp64-compare", action="store_true")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ inputs
+
+def haar(torch, n, g):
+    G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    Q, R = torch.linalg.qr(G)
+    del G
+    Q *= torch.sign(torch.diagonal(R))[None, :]
+    return Q
+
+
+def randsvd(torch, n, kappa, g, spd):
+    U = haar(torch, n, g)
+    V = U if spd else haar(torch, n, g)
+    sigma = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+    A = (U * sigma[None, :]) @ V.T
+    if spd:
+        A = (A + A.T) * 0.5
+    return A.contiguous()
+
+
+def spd_gram(torch, n, g):
+    G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    return (G @ G.T / n + torch.eye(n, dtype=torch.float64, device="cuda")).contiguous()
+
+
+def uniform(torch, shape, g):
+    return (torch.rand(*shape, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).contiguous()
+
+
+def backward_error(torch, A, X, B):
+    R = B - A @ X
+    num = R.abs().max(dim=0).values
+    den = A.abs().sum(1).max() * X.abs().max(dim=0).values + B.abs().max(dim=0).values
+    return (num / den).max().item()
+
+
+# ------------------------------------------------------------------ CPU baseline
+
+def cpu_baseline(workload, n, nrhs):
+    """The oracle (C restatement of the reference algorithm, one thread) on a
+    bounded sample of the workload, scaled to the metric's unit."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.lib()
+    rng = np.random.default_rng(7)
+    if workload == "gemm_fp64":
+        ns = 2048
+        A = rng.uniform(-1, 1, (ns, ns))
+        B = rng.uniform(-1, 1, (ns, ns))
+        t = time.perf_counter()
+        oracle.matMulP(A, B)
+        dt = time.perf_counter() - t
+        flops = 2.0 * ns ** 3
+        return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+                "sample": f"oracle matMulP {ns}x{ns}x{ns}, 1 run, 1 thread ({dt:.1f} s)"}
+    ns = 4096 if n >= 4096 else n
+    A = rng.uniform(-1, 1, (ns, ns))
+    b = rng.uniform(-1, 1, ns)
+    spd = workload.startswith("chol")
+    if spd:
+        G = rng.standard_normal((ns, ns))
+        A = G @ G.T / ns + np.eye(ns)
+    t = time.perf_counter()
+    (oracle.choleskySolveP if spd else oracle.luSolveP)(A, b)
+    dt = time.perf_counter() - t
+    # The reference API takes one right-hand side per call and re-factors on
+    # every call, so nrhs right-hand sides cost nrhs calls.
+    calls = max(nrhs, 1)
+    t_full = dt * (n / ns) ** 3 * calls
+    name = "choleskySolveP" if spd else "luSolveP"
+    return {"value": 1.0 / t_full, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"oracle {name} n={ns}, 1 RHS, 1 run, 1 thread ({dt:.1f} s), scaled by (n/{ns})^3 to n={n}"
+                      + (f" x {calls} calls (one per RHS, as the reference API)" if calls > 1 else "")}
+
+
+# ------------------------------------------------------------------ main
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(the path shards by problem, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import lmx
+
+    lmx.init(0)
+    wl = a.workload
+    defaults = {"lu_mixed": (16384, 1), "chol_mixed": (16384, 1), "lu_fp64": (8192, 16),
+                "chol_fp64": (8192, 16), "gemm_fp64": (8192, 0)}
+    n = a.n or defaults[wl][0]
+    nrhs = a.nrhs if a.nrhs is not None else defaults[wl][1]
+    g = torch.Generator(device="cuda").manual_seed(a.seed)
+    spd = wl.startswith("chol")
+    t0 = time.perf_counter()
+    if wl == "gemm_fp64":
+        A = uniform(torch, (n, n), g)
+        B = uniform(torch, (n, n), g)
+        C = torch.empty_like(A)
+    elif wl in ("lu_mixed", "chol_mixed"):
+        A = randsvd(torch, n, a.kappa, g, spd)
+        B = uniform(torch, (n, nrhs), g)
+        X = torch.zeros_like(B)
+    else:
+        A = spd_gram(torch, n, g) if spd else uniform(torch, (n, n), g)
+        B = uniform(torch, (n, nrhs), g)
+    torch.cuda.synchronize()
+    log(f"[bench] {wl} n={n} nrhs={nrhs}: inputs ready in {time.perf_counter() - t0:.1f} s")
+
+    # FP64 path work buffers
+    if wl != "gemm_fp64":
+        LU = torch.empty_like(A)
+        XB = torch.empty_like(B)
+        ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+    last = {}
+
+    def step():
+        if wl == "gemm_fp64":
+            lmx.dgemm_dev(A, B, C)
+        elif wl in ("lu_mixed", "chol_mixed"):
+            ri = (lmx.posv_mixed_dev if spd else lmx.gesv_mixed_dev)(A, B, X, lmx.PREC_F32)
+            last["ri"] = ri
+        else:
+            LU.copy_(A)
+            XB.copy_(B)
+            if spd:
+                lmx.posv_dev(LU, XB, info)
+            else:
+                lmx.gesv_dev(LU, ipiv, XB, info)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({a.warmup} steps)")
+
+    lmx.kernel_timing_reset()
+    lmx.kernel_timing(True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ev[0].record()
+    for i in range(a.steps):
+        step()
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    lmx.kernel_timing(False)
+    lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(a.steps)]
+    ms_per_step = wall * 1e3 / a.steps
+    log(f"[bench] timed {a.steps} steps: {ms_per_step:.2f} ms/step")
+
+    fam = {}
+    for f in KERNEL_FAMILIES + ["getrf", "potrf", "getrs", "refine"]:
+        cnt, ms, work = lmx.kernel_timing_get(f)
+        if cnt:
+            fam[f] = {"launches": cnt, "ms_per_step": ms / a.steps, "work_per_step": work / a.steps}
+    lmx.kernel_timing_reset()
+
+    # dominant MFMA kernel family and its roofline
+    mf = [f for f in ("gemm", "sgemm", "syrk") if f in fam]
+    dom = max(mf, key=lambda f: fam[f]["ms_per_step"]) if mf else None
+    roofline = None
+    if dom:
+        cnt, ms, work = fam[dom]["launches"], fam[dom]["ms_per_step"] * a.steps, fam[dom]["work_per_step"] * a.steps
+        achieved = work / (ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": {"gemm": "gemm_mfma_kernel<double>", "sgemm": "gemm_mfma_kernel<float>",
+                                                "syrk": "gemm_mfma_kernel<double> (lower tiles)"}[dom],
+                    "achieved": round(achieved, 3), "peak": PEAK[dom], "unit": "TFLOP/s",
+                    "frac": round(achieved / PEAK[dom], 4), "traffic": None,
+                    "launches_per_step": cnt // a.steps, "avg_launch_us": round(ms * 1e3 / cnt, 2)}
+
+    result = {"metric": "FP64-accurate solves/sec & TFLOP/s vs plain FP64 GPU; p50/p99 solve latency",
+              "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    lat_sorted = sorted(lat)
+    p50 = float(np.percentile(lat_sorted, 50))
+    p99 = float(np.percentile(lat_sorted, 99))
+
+    if wl == "gemm_fp64":
+        flops = 2.0 * n ** 3
+        result.update({"value": round(flops / (ms_per_step * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "dtype": "f64",
+                       "config": {"workload": f"GEMM C = A B, {n}x{n}x{n}, FP64 MFMA, A, B iid U[-1,1]", "n": n}})
+    else:
+        if spd:
+            flops = n ** 3 / 3.0 + 2.0 * n * n * nrhs
+        else:
+            flops = 2.0 * n ** 3 / 3.0 + 2.0 * n * n * nrhs
+        sps = 1e3 / ms_per_step
+        out = X if wl.endswith("mixed") else XB
+        eta = backward_error(torch, A, out, B)
+        name = "choleskySolve" if spd else "luSolve"
+        result.update({"value": round(sps, 4), "unit": "solves/s", "tflops": round(flops * sps / 1e12, 3),
+                       "p50_ms": round(p50, 3), "p99_ms": round(p99, 3), "backward_error": eta,
+                       "backward_error_bound_10neps": 10 * n * EPS})
+        if wl.endswith("mixed"):
+            ri = last["ri"]
+            result["dtype"] = "f32+f64"
+            result["refine"] = {"iterations": ri.iterations, "converged": ri.converged, "fell_back": ri.fell_back}
+            result["config"] = {"workload": f"{name} FP32-factor + FP64 refinement, n={n}, {nrhs} RHS, "
+                                            f"A = U diag(sigma) V^T (Haar, kappa={a.kappa:g})",
+                                "n": n, "nrhs": nrhs, "kappa": a.kappa, "factor": "fp32"}
+            if not a.no_fp64_compare:
+                k2 = max(2, a.steps // 2)
+                for _ in range(1):
+                    LU.copy_(A); XB.copy_(B)
+                    (lmx.posv_dev(LU, XB, info) if spd else lmx.gesv_dev(LU, ipiv, XB, info))
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                for _ in range(k2):
+                    LU.copy_(A); XB.copy_(B)
+                    (lmx.posv_dev(LU, XB, info) if spd else lmx.gesv_dev(LU, ipiv, XB, info))
+                torch.cuda.synchronize()
+                ms64 = (time.perf_counter() - t1) * 1e3 / k2
+                eta64 = backward_error(torch, A, XB, B)
+                result["fp64_path"] = {"solves_per_s": round(1e3 / ms64, 4), "ms_per_solve": round(ms64, 3),
+                                       "tflops": round(flops / (ms64 * 1e-3) / 1e12, 3), "backward_error": eta64}
+                result["speedup_vs_fp64"] = round(ms64 / ms_per_step, 3)
+                fwd = ((out - XB).abs().max() / XB.abs().max()).item()
+                result["rel_diff_vs_fp64_solution"] = fwd
+        else:
+            result["dtype"] = "f64"
+            result["config"] = {"workload": f"{name} FP64, n={n}, {nrhs} RHS, "
+                                            + ("A = G G^T/n + I" if spd else "A iid U[-1,1]"), "n": n, "nrhs": nrhs}
+    result["roofline"] = roofline
+    result["families"] = {k: {"launches_per_step": v["launches"] // a.steps, "ms_per_step": round(v["ms_per_step"], 3)}
+                          for k, v in fam.items()}
+    if not a.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle) ...")
+        try:
+            cpu = cpu_baseline(wl, n, nrhs)
+            cpu["cpu_model"] = platform.processor() or ""
+            try:
+                model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+                for line in model.splitlines():
+                    if line.startswith("Model name"):
+                        cpu["cpu_model"] = line.split(":", 1)[1].strip()
+            except OSError:
+                pass
+            cpu["host_cores"] = os.cpu_count()
+            result["cpu_baseline"] = cpu
+        except Exception as e:  # the baseline is reported, never required
+            result["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

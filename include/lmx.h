@@ -189,14 +189,16 @@ int lmx_dposv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs,
 
 /* When enabled, the library brackets every launch of the named kernel
  * family with HIP events on the launch stream and accumulates the elapsed
- * time.  Families: "gemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
- * "syrk", "getrs", "refine".  No reference counterpart. */
+ * time and the algorithmic work (flops) of each launch.  Kernel families:
+ * "gemm" (FP64 MFMA GEMM), "sgemm" (FP32), "syrk", "getrf_panel", "trsm",
+ * "laswp", "potrf_diag"; whole-call families: "getrf", "potrf", "getrs",
+ * "refine".  No reference counterpart. */
 int lmx_kernel_timing(int enable);
 int lmx_kernel_timing_reset(void);
-/* Count of timed launches and their total milliseconds for one family
- * (synchronises the device to read the events). */
+/* Count of timed launches, their total milliseconds and total work for one
+ * family (synchronises on the recorded events).  Pointers may be NULL. */
 int lmx_kernel_timing_get(const char *family, int64_t *count,
-                          double *total_ms);
+                          double *total_ms, double *total_work);
 
 #ifdef __cplusplus
 }

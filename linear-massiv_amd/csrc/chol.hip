@@ -89,6 +89,7 @@ void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info) {
     if (n > INT32_MAX / 2) fail_arg("potrf: n too large");
     clear_info(s, d_info);
     if (n == 0) return;
+    TimedRegion trall("potrf", s, (double)n * (double)n * (double)n / 3.0);
     potrf_blocked<T>(s, n, A, lda, 0, d_info, 128);
 }
 

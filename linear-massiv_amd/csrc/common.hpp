@@ -86,11 +86,14 @@ struct Scratch {
 
 // Optional per-family kernel timing (lmx_kernel_timing).
 bool timing_enabled();
+// `work` is the algorithmic work of the region (flops, or bytes for
+// memory-bound families), summed per family.
 struct TimedRegion {
     int slot = -1;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    TimedRegion(const char *family, hipStream_t st);
+    double work = 0;
+    TimedRegion(const char *family, hipStream_t st, double work = 0);
     ~TimedRegion();
 };
 

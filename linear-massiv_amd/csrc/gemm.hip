@@ -243,7 +243,7 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
     if (m == 0 || n == 0) return;
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
-    TimedRegion tr("gemm", s);
+    TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, 2.0 * (double)m * (double)n * (double)k);
     // Large outputs: 128 x 128 tiles.  Smaller ones: 64 x 64 tiles so the
     // launch still spreads over the 256 CUs.
     int64_t big_tiles = cdiv(m, 128) * cdiv(n, 128);
@@ -260,7 +260,7 @@ void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t
                int64_t ldc) {
     if (n <= 0) return;
     if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
-    TimedRegion tr("syrk", s);
+    TimedRegion tr("syrk", s, (double)n * (double)(n + 1) * (double)k);
     int64_t big_tiles = cdiv(n, 128) * (cdiv(n, 128) + 1) / 2;
     if (big_tiles >= 512)
         launch_tile<T, 128, 128, 16>(s, false, true, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,

@@ -443,6 +443,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
     if (n > INT32_MAX / 2) fail_arg("getrf: n too large");
     clear_info(s, d_info);
     if (n == 0) return;
+    TimedRegion trall("getrf", s, 2.0 / 3.0 * (double)n * (double)n * (double)n);
     const int N = (int)n;
     size_t sync_bytes = (sizeof(PanelSync<T>) + 255) & ~size_t(255);
     Scratch scr(sync_bytes + sizeof(int32_t) * ((size_t)N + 2 * KB + 8) + 256, s);

@@ -12,11 +12,12 @@ constexpr int kMaxDevices = 64;
 Context *g_ctx[kMaxDevices] = {};
 std::mutex g_ctx_mu;
 
-const char *kFamilies[] = {"gemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
-                           "syrk", "getrs", "refine"};
+const char *kFamilies[] = {"gemm", "sgemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
+                           "syrk", "getrs", "refine", "getrf", "potrf"};
 constexpr int kNumFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
 struct EventPair {
     hipEvent_t a, b;
+    double work;
 };
 std::mutex g_time_mu;
 bool g_timing = false;
@@ -102,7 +103,7 @@ Scratch::~Scratch() {
 
 bool timing_enabled() { return g_timing; }
 
-TimedRegion::TimedRegion(const char *family, hipStream_t st) : s(st) {
+TimedRegion::TimedRegion(const char *family, hipStream_t st, double w) : s(st), work(w) {
     if (!g_timing) return;
     std::lock_guard<std::mutex> g(g_time_mu);
     slot = family_slot(family);
@@ -115,7 +116,7 @@ TimedRegion::TimedRegion(const char *family, hipStream_t st) : s(st) {
 TimedRegion::~TimedRegion() {
     if (slot < 0) return;
     std::lock_guard<std::mutex> g(g_time_mu);
-    if (hipEventRecord(e1, s) == hipSuccess) g_pairs[slot].push_back({e0, e1});
+    if (hipEventRecord(e1, s) == hipSuccess) g_pairs[slot].push_back({e0, e1, work});
 }
 
 }  // namespace lmx
@@ -180,20 +181,22 @@ int lmx_kernel_timing_reset(void) {
     });
 }
 
-int lmx_kernel_timing_get(const char *family, int64_t *count, double *total_ms) {
+int lmx_kernel_timing_get(const char *family, int64_t *count, double *total_ms, double *total_work) {
     return guarded([&] {
         std::lock_guard<std::mutex> g(g_time_mu);
         int s = family_slot(family ? family : "");
         if (s < 0) fail_arg("unknown kernel family");
-        double tot = 0;
+        double tot = 0, work = 0;
         for (auto &p : g_pairs[s]) {
             LMX_HIP(hipEventSynchronize(p.b));
             float ms = 0;
             LMX_HIP(hipEventElapsedTime(&ms, p.a, p.b));
             tot += ms;
+            work += p.work;
         }
         if (count) *count = (int64_t)g_pairs[s].size();
         if (total_ms) *total_ms = tot;
+        if (total_work) *total_work = work;
         return LMX_OK;
     });
 }

@@ -103,7 +103,7 @@ _sigs = {
     "lmx_dposv_mixed": [ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _pRI],
     "lmx_dposv_mixed_dev": [_vp, ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _pRI],
     "lmx_kernel_timing": [ctypes.c_int], "lmx_kernel_timing_reset": [],
-    "lmx_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)],
+    "lmx_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)],
 }
 for _name, _args in _sigs.items():
     if not hasattr(_lib, _name):
@@ -435,7 +435,9 @@ def kernel_timing_reset():
 
 
 def kernel_timing_get(family: str):
+    """-> (launch count, total milliseconds, total algorithmic work)."""
     cnt = _i64(0)
     ms = _dbl(0)
-    _check(_lib.lmx_kernel_timing_get(family.encode(), ctypes.byref(cnt), ctypes.byref(ms)))
-    return cnt.value, ms.value
+    work = _dbl(0)
+    _check(_lib.lmx_kernel_timing_get(family.encode(), ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(work)))
+    return cnt.value, ms.value, work.value
