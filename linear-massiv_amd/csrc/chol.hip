@@ -28,27 +28,33 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
     __shared__ T L[DIAG][DIAG + 1];
     __shared__ int s_bad;
     const int t = threadIdx.x;
+#pragma unroll 16
     for (int e = t; e < nb * nb; e += 256) {
         int r = e / nb, c = e % nb;
         L[r][c] = (c <= r) ? A[(int64_t)r * lda + c] : T(0);
     }
     if (t == 0) s_bad = 0;
     __syncthreads();
+    // thread t updates row i = t % 64 at columns c = t / 64 + 4 q
+    const int i = t & 63, c0 = t >> 6;
     for (int j = 0; j < nb; ++j) {
+        const T d = L[j][j];
+        const T sjj = sqrt(d);
+        __syncthreads();  // every thread has read L[j][j]
         if (t == 0) {
-            T d = L[j][j];
             if (!(d > T(0)) && s_bad == 0) s_bad = col0 + j + 1;
-            L[j][j] = sqrt(d);
+            L[j][j] = sjj;
         }
-        __syncthreads();
-        const T djj = L[j][j];
-        for (int i = j + 1 + t; i < nb; i += 256) L[i][j] = L[i][j] / djj;
+        if (t > j && t < nb) L[t][j] = L[t][j] / sjj;
         __syncthreads();
         // trailing update of the lower triangle: L[i][c] -= L[i][j] L[c][j], j < c <= i
-        const int w = nb - j - 1;
-        for (int e = t; e < w * w; e += 256) {
-            int i = j + 1 + e / w, c = j + 1 + e % w;
-            if (c <= i) L[i][c] = fma(-L[i][j], L[c][j], L[i][c]);
+        if (i > j && i < nb) {
+            const T lij = L[i][j];
+#pragma unroll
+            for (int q = 0; q < DIAG / 4; ++q) {
+                int c = c0 + 4 * q;
+                if (c > j && c <= i) L[i][c] = fma(-lij, L[c][j], L[i][c]);
+            }
         }
         __syncthreads();
     }

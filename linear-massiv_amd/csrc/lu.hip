@@ -36,16 +36,22 @@ namespace {
 
 constexpr int KB = 32;        // panel kernel width
 constexpr int RPT = 2;        // rows per thread in the panel kernel
-constexpr int PANEL_ROWS = 256 * RPT;
-constexpr int MAXP = 256;     // max cooperative workgroups
+constexpr int PT = 512;       // threads per panel workgroup (8 waves)
+constexpr int PW = PT / 64;   // waves per panel workgroup
+constexpr int PANEL_ROWS = PT * RPT;
+constexpr int MAXP = 64;      // max cooperative workgroups (panel height <= 32768)
 constexpr int NB_OUTER = 128; // outer panel width (k of the trailing GEMM)
 constexpr long SPIN_LIMIT = 1L << 22;
 
+// Cross-workgroup state of the panel kernel.  seq[w] counts the columns
+// workgroup w has published (monotonic over a whole factorisation, so it
+// needs clearing once per getrf); candidates are double-buffered by column
+// parity.
 template <class T>
 struct PanelSync {
-    unsigned counter;
+    unsigned seq[MAXP];
     unsigned error;
-    unsigned pad[14];
+    unsigned pad[15];
     T cand_val[2][MAXP];
     int cand_row[2][MAXP];
     int cand_tag[2][MAXP];
@@ -53,24 +59,51 @@ struct PanelSync {
     T diag_data[2][KB];
     int diag_tag[2];
 };
+constexpr int SYNC_CLEAR_WORDS = MAXP + 16;
 
-template <class T>
-__device__ inline void st(T *p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Wave-wide max / min on DPP lane permutations (no LDS round trip): quad
+// swaps, row rotations by 4 and 8, then the row broadcasts of lanes 15 and
+// 31; the full result lands in lane 63.  fmax ignores NaN, so a NaN never
+// wins the pivot search (as with rawPivotSearch's strict '>').
+template <int CTRL, int RM>
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(dpp_i<CTRL, RM>(__float_as_int(v)));
+}
+template <int CTRL, int RM>
+__device__ inline double dpp_f(double v) {
+    return __hiloint2double(dpp_i<CTRL, RM>(__double2hiint(v)), dpp_i<CTRL, RM>(__double2loint(v)));
+}
+__device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ inline double lane63(double v) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 template <class T>
-__device__ inline T ld(const T *p) {
-    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline T wave_max(T v) {
+    v = fmax(v, dpp_f<0xB1, 0xF>(v));   // quad_perm 1,0,3,2
+    v = fmax(v, dpp_f<0x4E, 0xF>(v));   // quad_perm 2,3,0,1
+    v = fmax(v, dpp_f<0x124, 0xF>(v));  // row_ror 4
+    v = fmax(v, dpp_f<0x128, 0xF>(v));  // row_ror 8
+    v = fmax(v, dpp_f<0x142, 0xA>(v));  // row_bcast 15
+    v = fmax(v, dpp_f<0x143, 0xC>(v));  // row_bcast 31
+    return lane63(v);
+}
+__device__ inline int wave_min(int v) {
+    v = min(v, dpp_i<0xB1, 0xF>(v));
+    v = min(v, dpp_i<0x4E, 0xF>(v));
+    v = min(v, dpp_i<0x124, 0xF>(v));
+    v = min(v, dpp_i<0x128, 0xF>(v));
+    v = min(v, dpp_i<0x142, 0xA>(v));
+    v = min(v, dpp_i<0x143, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
-// (a, row) beats (b, brow): larger magnitude, ties to the lower row; NaN never wins.
 template <class T>
-__device__ inline bool better(T a, int row, T b, int brow) {
-    return a > b || (a == b && row < brow);
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
+__global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
                                                           int k0, int32_t *__restrict__ ipiv,
                                                           int32_t *__restrict__ srcmap,
                                                           int32_t *__restrict__ moves,
@@ -78,157 +111,147 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int kb, T *__re
                                                           PanelSync<T> *__restrict__ sy, int P,
                                                           unsigned base) {
     const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    __shared__ T s_prow[KB], s_drow[KB];
-    __shared__ int s_ptag, s_dtag, s_p, s_pw;
-    __shared__ T s_wbest[4];
-    __shared__ int s_wrow[4];
+    __shared__ T s_cand[KB], s_drow[KB], s_prow[KB];
+    __shared__ int s_ctag, s_dtag;
+    __shared__ T s_wbest[PW];
+    __shared__ int s_wrow[PW];
     __shared__ int s_pivs[KB];
+    __shared__ T s_val[MAXP];
+    __shared__ int s_row[MAXP], s_tag[MAXP];
+    __shared__ T s_all[MAXP][KB + 1];
 
     T v[RPT][KB];
     int tag[RPT], row[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        row[q] = w * PANEL_ROWS + q * 256 + t;
+        row[q] = w * PANEL_ROWS + q * PT + t;
         tag[q] = row[q];
 #pragma unroll
         for (int c = 0; c < KB; ++c)
             v[q][c] = (row[q] < m && c < kb) ? A[(int64_t)row[q] * lda + c] : T(0);
     }
     int local_info = 0;
+    bool failed = false;
 
     for (int j = 0; j < kb; ++j) {
         const int par = j & 1;
-        // 1. candidate of this workgroup
-        T best = T(-1);
-        int brow = INT_MAX;
+        // 1. candidate of this workgroup: max |a_ij| over its rows i >= j,
+        //    then the lowest row attaining it (wave DPP reductions, then
+        //    one LDS exchange between the waves)
+        T a[RPT];
+        T tmax = T(-1);
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            a[q] = (row[q] >= j && row[q] < m) ? fabs(v[q][0]) : T(-1);
+            tmax = fmax(tmax, a[q]);
+        }
+        const T wmax = wave_max(tmax);
+        int trow = INT_MAX;
 #pragma unroll
         for (int q = 0; q < RPT; ++q)
-            if (row[q] >= j && row[q] < m) {
-                T a = fabs(v[q][0]);
-                if (better(a, row[q], best, brow)) {
-                    best = a;
-                    brow = row[q];
-                }
-            }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            T ob = __shfl_xor(best, off);
-            int orow = __shfl_xor(brow, off);
-            if (better(ob, orow, best, brow)) {
-                best = ob;
-                brow = orow;
-            }
-        }
+            if (a[q] == wmax && row[q] >= j && row[q] < m) trow = min(trow, row[q]);
+        const int wrow = wave_min(trow);
         if (lane == 0) {
-            s_wbest[wave] = best;
-            s_wrow[wave] = brow;
+            s_wbest[wave] = wmax;
+            s_wrow[wave] = wrow;
         }
         __syncthreads();
-        best = s_wbest[0];
-        brow = s_wrow[0];
+        T best = s_wbest[0];
 #pragma unroll
-        for (int x = 1; x < 4; ++x)
-            if (better(s_wbest[x], s_wrow[x], best, brow)) {
-                best = s_wbest[x];
-                brow = s_wrow[x];
+        for (int x = 1; x < PW; ++x) best = fmax(best, s_wbest[x]);
+        int brow = INT_MAX;
+#pragma unroll
+        for (int x = 0; x < PW; ++x)
+            if (s_wbest[x] == best) brow = min(brow, s_wrow[x]);
+        if (!(best >= T(0))) brow = INT_MAX;  // no row of this workgroup is a candidate
+        // 2. stage the candidate row and row j (always in workgroup 0) in LDS
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (row[q] == brow) {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) s_cand[c] = v[q][c];
+                s_ctag = tag[q];
             }
+            if (row[q] == j) {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) s_drow[c] = v[q][c];
+                s_dtag = tag[q];
+            }
+        }
+        __syncthreads();
 
-        int p;
+        int p, ptag;
         if (P > 1) {
-            // 2. publish and synchronise
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                if (row[q] == brow) {
-#pragma unroll
-                    for (int c = 0; c < KB; ++c) st(&sy->cand_data[par][w][c], v[q][c]);
-                    st(&sy->cand_tag[par][w], tag[q]);
+            // 3. publish: plain stores by wave 0, one release, then the sequence number
+            if (wave == 0) {
+                if (lane < KB) sy->cand_data[par][w][lane] = s_cand[lane];
+                if (lane == KB) sy->cand_val[par][w] = best;
+                if (lane == KB + 1) sy->cand_row[par][w] = brow;
+                if (lane == KB + 2) sy->cand_tag[par][w] = s_ctag;
+                if (w == 0) {
+                    if (lane < KB) sy->diag_data[par][lane] = s_drow[lane];
+                    if (lane == KB + 3) sy->diag_tag[par] = s_dtag;
                 }
-                if (row[q] == j) {
-#pragma unroll
-                    for (int c = 0; c < KB; ++c) st(&sy->diag_data[par][c], v[q][c]);
-                    st(&sy->diag_tag[par], tag[q]);
-                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (lane == 0)
+                    __hip_atomic_store(&sy->seq[w], base + (unsigned)j + 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (t == 0) {
-                st(&sy->cand_val[par][w], best);
-                st(&sy->cand_row[par][w], brow);
-            }
-            __threadfence();
-            __syncthreads();
-            if (t == 0) {
-                __hip_atomic_fetch_add(&sy->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned target = base + (unsigned)(j + 1) * (unsigned)P;
+            // 4. wait until every workgroup has published column j
+            if (t < P && !failed) {
+                const unsigned want = base + (unsigned)j + 1u;
                 long spins = 0;
-                while (__hip_atomic_load(&sy->counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                    if (ld(&sy->error) != 0u) break;
+                while (__hip_atomic_load(&sy->seq[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+                    if (__hip_atomic_load(&sy->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                        failed = true;
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > SPIN_LIMIT) {
-                        st(&sy->error, 1u);
+                        __hip_atomic_store(&sy->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        failed = true;
                         break;
                     }
                 }
             }
             __syncthreads();
-            // 3. global pivot
-            if (wave == 0) {
-                T b = T(-1);
-                int r = INT_MAX, ws = -1;
-                for (int x = lane; x < P; x += 64) {
-                    T cb = ld(&sy->cand_val[par][x]);
-                    int cr = ld(&sy->cand_row[par][x]);
-                    if (better(cb, cr, b, r)) {
-                        b = cb;
-                        r = cr;
-                        ws = x;
-                    }
-                }
-#pragma unroll
-                for (int off = 32; off >= 1; off >>= 1) {
-                    T ob = __shfl_xor(b, off);
-                    int orow = __shfl_xor(r, off);
-                    int ow = __shfl_xor(ws, off);
-                    if (better(ob, orow, b, r)) {
-                        b = ob;
-                        r = orow;
-                        ws = ow;
-                    }
-                }
-                if (lane == 0) {
-                    s_p = (r == INT_MAX) ? j : r;
-                    s_pw = (r == INT_MAX) ? -1 : ws;
-                }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // 5. fetch every candidate record in one round of plain loads
+            for (int e = t; e < P; e += PT) {
+                s_val[e] = sy->cand_val[par][e];
+                s_row[e] = sy->cand_row[par][e];
+                s_tag[e] = sy->cand_tag[par][e];
+            }
+            for (int e = t; e < P * KB; e += PT) s_all[e / KB][e % KB] = sy->cand_data[par][e / KB][e % KB];
+            if (w != 0) {
+                if (t < KB) s_drow[t] = sy->diag_data[par][t];
+                if (t == 0) s_dtag = sy->diag_tag[par];
             }
             __syncthreads();
-            p = s_p;
-            const int pw = s_pw;
-            if (t < KB) {
-                s_prow[t] = (pw >= 0) ? ld(&sy->cand_data[par][pw][t]) : ld(&sy->diag_data[par][t]);
-                s_drow[t] = ld(&sy->diag_data[par][t]);
-            }
-            if (t == 0) {
-                s_ptag = (pw >= 0) ? ld(&sy->cand_tag[par][pw]) : ld(&sy->diag_tag[par]);
-                s_dtag = ld(&sy->diag_tag[par]);
-            }
-            __syncthreads();
+            // 6. the global pivot, from LDS (same rule as within a workgroup)
+            T b = T(-1);
+            for (int x = 0; x < P; ++x) b = fmax(b, s_val[x]);
+            int r = INT_MAX, ws = -1;
+            for (int x = 0; x < P; ++x)
+                if (b >= T(0) && s_val[x] == b && s_row[x] < r) {
+                    r = s_row[x];
+                    ws = x;
+                }
+            // a NaN at row j keeps row j, as rawPivotSearch does
+            if (isnan(s_drow[0])) ws = -1;
+            p = (ws < 0) ? j : r;
+            if (t < KB) s_prow[t] = (ws >= 0) ? s_all[ws][t] : s_drow[t];
+            ptag = (ws >= 0) ? s_tag[ws] : s_dtag;
         } else {
-            p = (brow == INT_MAX) ? j : brow;
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                if (row[q] == p) {
-#pragma unroll
-                    for (int c = 0; c < KB; ++c) s_prow[c] = v[q][c];
-                    s_ptag = tag[q];
-                }
-                if (row[q] == j) {
-#pragma unroll
-                    for (int c = 0; c < KB; ++c) s_drow[c] = v[q][c];
-                    s_dtag = tag[q];
-                }
-            }
-            __syncthreads();
+            const bool own = brow != INT_MAX && !isnan(s_drow[0]);
+            p = own ? brow : j;
+            if (t < KB) s_prow[t] = own ? s_cand[t] : s_drow[t];
+            ptag = own ? s_ctag : s_dtag;
         }
+        __syncthreads();
+        const T *prow = s_prow;
 
-        // 4. interchange rows j and p
+        // 7. interchange rows j and p
         if (p != j) {
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
@@ -239,15 +262,15 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int kb, T *__re
                 } else if (row[q] == j) {
 #pragma unroll
                     for (int c = 0; c < KB; ++c) v[q][c] = s_prow[c];
-                    tag[q] = s_ptag;
+                    tag[q] = ptag;
                 }
             }
         }
         if (t == 0) s_pivs[j] = p;
         if (w == 0 && t == 0) ipiv[k0 + j] = k0 + p;
 
-        // 5. multipliers and rank-1 update of the remaining panel columns
-        const T piv = s_prow[0];
+        // 8. multipliers and rank-1 update of the remaining panel columns
+        const T piv = prow[0];
         if (piv != T(0)) {
             const int lim = kb - j;
 #pragma unroll
@@ -257,12 +280,12 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int kb, T *__re
                     v[q][0] = l;
 #pragma unroll
                     for (int c = 1; c < KB; ++c)
-                        if (c < lim) v[q][c] = fma(-l, s_prow[c], v[q][c]);
+                        if (c < lim) v[q][c] = fma(-l, prow[c], v[q][c]);
                 }
         } else if (local_info == 0) {
             local_info = k0 + j + 1;
         }
-        // 6. rotate: the next column moves to register 0
+        // 9. rotate: the next column moves to register 0
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             T f = v[q][0];
@@ -298,6 +321,7 @@ __global__ __launch_bounds__(256) void getf2_panel_kernel(int m, int kb, T *__re
         if (local_info != 0 && *info == 0) *info = local_info;
     }
 }
+
 
 // X[d] <- X[src[d]] for every d of the move list, for columns [c0, c1) and
 // [c2, c3) of X (rows relative to X).  All reads of a column finish before
@@ -407,14 +431,14 @@ void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32
     if (P > MAXP) fail_arg("getrf: matrix too tall for the panel kernel");
     TimedRegion tr("getrf_panel", s);
     if (P == 1) {
-        hipLaunchKernelGGL(getf2_panel_kernel<T>, dim3(1), dim3(256), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
+        hipLaunchKernelGGL(getf2_panel_kernel<T>, dim3(1), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
                            moves, info, sy, 1, base);
         LMX_LAUNCH_CHECK();
-        return;
+    } else {
+        void *args[] = {&m, &kb, &A, &lda, &k0, &ipiv, &srcmap, &moves, &info, &sy, &P, &base};
+        LMX_HIP(hipLaunchCooperativeKernel((const void *)getf2_panel_kernel<T>, dim3(P), dim3(PT), args, 0, s));
     }
-    void *args[] = {&m, &kb, &A, &lda, &k0, &ipiv, &srcmap, &moves, &info, &sy, &P, &base};
-    LMX_HIP(hipLaunchCooperativeKernel((const void *)getf2_panel_kernel<T>, dim3(P), dim3(256), args, 0, s));
-    base += (unsigned)P * (unsigned)kb;
+    base += (unsigned)kb;  // sequence numbers stay monotonic across launches
 }
 
 template <class T>
@@ -450,7 +474,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
     auto *sy = scr.as<PanelSync<T>>();
     int32_t *srcmap = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scr.p) + sync_bytes);
     int32_t *moves = srcmap + N;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, s, (unsigned *)sy, 16);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, s, (unsigned *)sy, SYNC_CLEAR_WORDS);
     LMX_LAUNCH_CHECK();
     if (d_perm) {
         hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, s, d_perm, N);

@@ -74,16 +74,87 @@ __global__ __launch_bounds__(256) void norm_inf_rows_kernel(const double *__rest
     if (q == 0 && i < n) atomic_max_nonneg(out, part[0][r] + part[1][r] + part[2][r] + part[3][r]);
 }
 
-// out[c] = max_i |X[i][c]| for an n x ncols block.
+// out[c] = max_i |X[i][c]| for an n x ncols block: a workgroup covers
+// COLMAX_ROWS rows and 64 columns (blockIdx.y), reduces in LDS, and issues
+// one atomic per column.
+constexpr int COLMAX_ROWS = 1024;
 __global__ __launch_bounds__(256) void colmax_kernel(const double *__restrict__ X, int64_t ldx, int64_t n,
                                                      int ncols, double *__restrict__ out) {
-    const int64_t e0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t total = n * ncols;
-    for (int64_t e = e0; e < total; e += (int64_t)gridDim.x * 256) {
-        int64_t i = e / ncols;
-        int c = (int)(e % ncols);
-        atomic_max_nonneg(&out[c], fabs(X[i * ldx + c]));
+    __shared__ double part[4][64];
+    const int t = threadIdx.x, cl = t & 63, q = t >> 6;
+    const int c = blockIdx.y * 64 + cl;
+    const int64_t r0 = (int64_t)blockIdx.x * COLMAX_ROWS;
+    double m = 0.0;
+    if (c < ncols)
+        for (int64_t i = r0 + q; i < n && i < r0 + COLMAX_ROWS; i += 4) {
+            double v = fabs(X[i * ldx + c]);
+            if (!(v <= m)) m = v;  // keeps NaN
+        }
+    part[q][cl] = m;
+    __syncthreads();
+    if (q == 0 && c < ncols) {
+        double r = part[0][cl];
+        for (int x = 1; x < 4; ++x)
+            if (!(part[x][cl] <= r)) r = part[x][cl];
+        atomic_max_nonneg(&out[c], r);
     }
+}
+
+// R = B - A X in FP64 for nrhs <= NR: one wave per row of A (coalesced
+// row reads, many rows in flight), wave-shuffle reduction.
+template <int NR>
+__global__ __launch_bounds__(256) void resid_kernel(int64_t n, int nrhs, const double *__restrict__ A, int64_t lda,
+                                                    const double *__restrict__ X, int64_t ldx,
+                                                    const double *__restrict__ B, int64_t ldb,
+                                                    double *__restrict__ R, int64_t ldr) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    double acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = 0.0;
+    const double *a = A + i * lda;
+#pragma unroll 4
+    for (int64_t k = lane; k < n; k += 64) {
+        double av = a[k];
+#pragma unroll
+        for (int c = 0; c < NR; ++c)
+            if (c < nrhs) acc[c] = fma(av, X[k * ldx + c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+        double v = acc[c];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        acc[c] = v;
+    }
+    if (lane < nrhs && lane < NR) {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < NR; ++c)
+            if (c == lane) v = acc[c];
+        R[i * ldr + lane] = B[i * ldb + lane] - v;
+    }
+}
+
+void colmax(hipStream_t s, const double *X, int64_t ldx, int64_t n, int ncols, double *out) {
+    dim3 g((unsigned)cdiv(n, COLMAX_ROWS), (unsigned)cdiv(ncols, 64));
+    hipLaunchKernelGGL(colmax_kernel, g, dim3(256), 0, s, X, ldx, n, ncols, out);
+    LMX_LAUNCH_CHECK();
+}
+
+// R = B - A X (FP64).
+void residual(hipStream_t s, int64_t n, int64_t k, const double *A, int64_t lda, const double *X, int64_t ldx,
+              const double *B, int64_t ldb, double *R, int64_t ldr) {
+    if (k <= 4) {
+        hipLaunchKernelGGL(resid_kernel<4>, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, n, (int)k, A, lda, X, ldx,
+                           B, ldb, R, ldr);
+        LMX_LAUNCH_CHECK();
+        return;
+    }
+    hipLaunchKernelGGL(copy_d_kernel, dim3((unsigned)cdiv(n * k, 256)), dim3(256), 0, s, B, ldb, R, ldr, n, k);
+    LMX_LAUNCH_CHECK();
+    gemm(s, false, false, n, k, n, -1.0, A, lda, X, ldx, 1.0, R, ldr);
 }
 
 // norms layout: [0] ||A||, [1..1+k) ||b_c||, [1+k..1+2k) ||x_c||, [1+2k..1+3k) ||r_c||,
@@ -101,8 +172,6 @@ __global__ void eta_kernel(double *norms, int k, double crit) {
     norms[1 + 3 * k] = eta;
     norms[2 + 3 * k] = pass;
 }
-
-unsigned grid_for(int64_t elems) { return (unsigned)std::min<int64_t>(cdiv(elems, 256), 1 << 20); }
 
 template <class F>
 void launch1d(hipStream_t s, int64_t elems, F launch) {
@@ -153,7 +222,7 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
         LMX_LAUNCH_CHECK();
         hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, A, lda, n, norms);
         LMX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, B, ldb, n, K, norms + 1);
+        colmax(s, B, ldb, n, K, norms + 1);
         LMX_LAUNCH_CHECK();
         // R = B
         launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
@@ -168,11 +237,10 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
                 hipLaunchKernelGGL(f2d_acc_kernel, g, dim3(256), 0, s, R32, k, X, ldx, n, k, it == 0 ? 1 : 0);
             });
             // R = B - A X
-            launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
-            gemm(s, false, false, n, k, n, -1.0, A, lda, X, ldx, 1.0, R, k);
+            residual(s, n, k, A, lda, X, ldx, B, ldb, R, k);
             hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(2 * K + 2, 256)), dim3(256), 0, s, norms + 1 + K, 2 * K + 2);
-            hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, X, ldx, n, K, norms + 1 + K);
-            hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, R, k, n, K, norms + 1 + 2 * K);
+            colmax(s, X, ldx, n, K, norms + 1 + K);
+            colmax(s, R, k, n, K, norms + 1 + 2 * K);
             hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
             LMX_LAUNCH_CHECK();
             double h[2];
@@ -203,13 +271,12 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
             getrs_perm<double>(s, n, k, A64, n, perm, X, ldx);
         }
         // report the backward error of the FP64 solve
-        launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
-        gemm(s, false, false, n, k, n, -1.0, A, lda, X, ldx, 1.0, R, k);
+        residual(s, n, k, A, lda, X, ldx, B, ldb, R, k);
         hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
         hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, A, lda, n, norms);
-        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, B, ldb, n, K, norms + 1);
-        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, X, ldx, n, K, norms + 1 + K);
-        hipLaunchKernelGGL(colmax_kernel, dim3(grid_for((int64_t)nk)), dim3(256), 0, s, R, k, n, K, norms + 1 + 2 * K);
+        colmax(s, B, ldb, n, K, norms + 1);
+        colmax(s, X, ldx, n, K, norms + 1 + K);
+        colmax(s, R, k, n, K, norms + 1 + 2 * K);
         hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
         LMX_LAUNCH_CHECK();
         double h[2];

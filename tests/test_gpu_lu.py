@@ -24,7 +24,7 @@ def test_getrf_vs_lapack_golden(lmx_gpu, orc, golden, n):
     np.testing.assert_array_equal(ipiv, swo)
 
 
-@pytest.mark.parametrize("n", [200, 511, 512, 513, 1000])
+@pytest.mark.parametrize("n", [200, 511, 512, 513, 1000, 1100, 2100])
 def test_getrf_vs_oracle(lmx_gpu, orc, n):
     rng = np.random.default_rng(n)
     A = rng.uniform(-1, 1, (n, n))
@@ -106,6 +106,25 @@ def test_tie_breaking_first_max(lmx_gpu, orc):
     LU, ipiv, info = lmx_gpu.getrf(A)
     assert ipiv[0] == 3
     np.testing.assert_array_equal(ipiv, orc.lu_p_factor(A)[1])
+
+
+def test_nan_pivot_rule(lmx_gpu, orc):
+    # rawPivotSearch (Internal/Kernel.hs:1541-1556) never selects a NaN below
+    # row k, and keeps row k when a_kk itself is NaN
+    n = 70
+    rng = np.random.default_rng(8)
+    A = rng.uniform(-1, 1, (n, n))
+    A[5, 0] = np.nan
+    A[9, 0] = 2.0
+    _, ipiv, _ = lmx_gpu.getrf(A)
+    assert ipiv[0] == 9
+    assert ipiv[0] == orc.lu_p_factor(A)[1][0]
+    B = rng.uniform(-1, 1, (n, n))
+    B[0, 0] = np.nan
+    B[3, 0] = 5.0
+    _, ipiv, _ = lmx_gpu.getrf(B)
+    assert ipiv[0] == 0
+    assert ipiv[0] == orc.lu_p_factor(B)[1][0]
 
 
 @pytest.mark.parametrize("n,zero_col", [(5, 2), (100, 40), (700, 650)])
