@@ -9,8 +9,12 @@ refinement) with A, b already resident in HBM.  The same inputs are also
 solved by the plain FP64 path (FP64 factor + solve) for the "vs plain FP64
 GPU" part of the metric.
 
-Prints ONE JSON line.  Other workloads (--workload) exist for profiling:
-lu_fp64 / chol_fp64 (n = 8192, 16 RHS), gemm_fp64 (8192^3), chol_mixed.
+Prints ONE JSON line.  Other workloads (--workload): lu_fp64 / chol_fp64
+(n = 8192, 16 RHS), gemm_fp64 (8192^3), chol_mixed, and the batched
+workloads batch_mixed / batch_fp64: 20,000 LU (A iid U[-1,1]) plus 20,000
+Cholesky (A = G G^T/n + I) independent solves, n log-uniform in [8, 512],
+1-8 right-hand sides each, in one persistent launch (one step = the whole
+batch).
 """
 import argparse
 import json
@@ -34,7 +38,10 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="lu_mixed",
-                   choices=["lu_mixed", "chol_mixed", "lu_fp64", "chol_fp64", "gemm_fp64"])
+                   choices=["lu_mixed", "chol_mixed", "lu_fp64", "chol_fp64", "gemm_fp64", "batch_mixed",
+                            "batch_fp64"])
+    p.add_argument("--count", type=int, default=20000, help="problems per family (batched workloads)")
+    p.add_argument("--autotune", action="store_true", help="batched workloads: autotune the kernel variant")
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--nrhs", type=int, default=None)
     p.add_argument("--kappa", type=float, default=1e6)
@@ -86,7 +93,7 @@ def backward_error(torch, A, X, B):
 
 # ------------------------------------------------------------------ CPU baseline
 
-def cpu_baseline(workload, n, nrhs):
+def cpu_baseline(workload, n, nrhs, batch=None):
     """The oracle (C restatement of the reference algorithm, one thread) on a
     bounded sample of the workload, scaled to the metric's unit."""
     import numpy as np
@@ -94,6 +101,31 @@ def cpu_baseline(workload, n, nrhs):
     import oracle
     oracle.lib()
     rng = np.random.default_rng(7)
+    if batch is not None:
+        # a random sample of the batch's problems; the reference API solves
+        # one right-hand side per call (and factors on every call)
+        shapes, total_flops = batch
+        idx = rng.choice(len(shapes), size=min(300, len(shapes)), replace=False)
+        t_used, f_used = 0.0, 0.0
+        for i in idx:
+            kind, m, r = shapes[i]
+            if kind == 0:
+                A = rng.uniform(-1, 1, (m, m))
+            else:
+                G = rng.standard_normal((m, m))
+                A = G @ G.T / m + np.eye(m)
+            b = rng.uniform(-1, 1, m)
+            t = time.perf_counter()
+            for _ in range(r):
+                (oracle.choleskySolveP if kind else oracle.luSolveP)(A, b)
+            t_used += time.perf_counter() - t
+            f_used += (m ** 3 / 3.0 if kind else 2.0 * m ** 3 / 3.0) + 2.0 * m * m * r
+            if t_used > 20:
+                break
+        t_full = t_used * total_flops / f_used
+        return {"value": len(shapes) / t_full, "unit": "solves/s", "cores": 1, "kind": "port",
+                "sample": f"oracle luSolveP / choleskySolveP on {len(idx)} random problems of the batch, one call "
+                          f"per RHS, 1 thread ({t_used:.1f} s), scaled by flops to the {len(shapes)} problems"}
     if workload == "gemm_fp64":
         ns = 2048
         A = rng.uniform(-1, 1, (ns, ns))
@@ -124,6 +156,146 @@ def cpu_baseline(workload, n, nrhs):
                       + (f" x {calls} calls (one per RHS, as the reference API)" if calls > 1 else "")}
 
 
+# ------------------------------------------------------------------ batched
+
+def batch_inputs(torch, lmx, np, count, seed):
+    """count LU and count Cholesky problems, n log-uniform in [8, 512], 1-8
+    RHS; matrices generated on the device by size group."""
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for kind in (0, 1):
+        ns = np.exp(rng.uniform(np.log(8), np.log(512), count)).round().astype(int).clip(8, 512)
+        rs = rng.integers(1, 9, count)
+        shapes += [(kind, int(m), int(r)) for m, r in zip(ns, rs)]
+    shapes.sort(key=lambda x: (x[0], x[1]))  # size groups are contiguous in the pools
+    probs, na, nb, npv = lmx.batch_layout(shapes)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.empty(na, dtype=torch.float64, device="cuda")
+    i = 0
+    while i < len(shapes):
+        kind, m = shapes[i][0], shapes[i][1]
+        j = i
+        while j < len(shapes) and shapes[j][0] == kind and shapes[j][1] == m:
+            j += 1
+        c = j - i
+        off = probs[i].a_offset
+        if kind == 0:
+            blk = torch.rand(c, m, m, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        else:
+            G = torch.randn(c, m, m, dtype=torch.float64, device="cuda", generator=g)
+            blk = torch.bmm(G, G.transpose(1, 2).contiguous()) / m + torch.eye(m, dtype=torch.float64,
+                                                                                  device="cuda")
+            del G
+        A[off:off + c * m * m] = blk.reshape(-1)
+        del blk
+        i = j
+    B = torch.rand(nb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    return shapes, probs, na, nb, npv, A, B
+
+
+def batch_backward_error(torch, probs, A, X, B):
+    """max over problems of eta / (n eps), grouped by (n, nrhs) on the device."""
+    groups = {}
+    for p in probs:
+        groups.setdefault((p.n, p.nrhs), []).append(p)
+    worst = 0.0
+    for (m, r), ps in groups.items():
+        ia = torch.tensor([p.a_offset for p in ps], device="cuda")[:, None] + torch.arange(m * m, device="cuda")
+        ib = torch.tensor([p.b_offset for p in ps], device="cuda")[:, None] + torch.arange(m * r, device="cuda")
+        Am = A[ia].view(-1, m, m)
+        kinds = torch.tensor([p.kind for p in ps], device="cuda")
+        Al = torch.tril(Am)
+        Asym = Al + torch.tril(Am, -1).transpose(1, 2)
+        Am = torch.where((kinds == 1)[:, None, None], Asym, Am)
+        Xm = X[ib].view(-1, m, r)
+        Bm = B[ib].view(-1, m, r)
+        R = Bm - torch.bmm(Am, Xm.contiguous())
+        num = R.abs().amax(1)
+        den = Am.abs().sum(2).amax(1)[:, None] * Xm.abs().amax(1) + Bm.abs().amax(1)
+        worst = max(worst, (num / den).max().item() / (m * EPS))
+    return worst
+
+
+def bench_batch(a, np, torch, lmx):
+    mixed = a.workload == "batch_mixed"
+    t0 = time.perf_counter()
+    shapes, probs, na, nb, npv, A, B = batch_inputs(torch, lmx, np, a.count, a.seed)
+    torch.cuda.synchronize()
+    log(f"[bench] {a.workload}: {len(shapes)} problems, A pool {na * 8 / 2**30:.2f} GiB, inputs ready in "
+        f"{time.perf_counter() - t0:.1f} s")
+    flags = lmx.BATCH_GRAPH | (lmx.BATCH_AUTOTUNE if a.autotune else 0)
+    prec = lmx.PREC_F32 if mixed else lmx.PREC_F64
+    plan = lmx.BatchPlan(probs, na, nb, 0, prec, flags)
+    q = plan.query()
+    log(f"[bench] plan: {q}")
+    A0 = A.clone() if not mixed else A
+    Bw = B.clone()
+    info = torch.zeros(len(shapes), dtype=torch.int32, device="cuda")
+    iters = torch.zeros(len(shapes), dtype=torch.int32, device="cuda")
+
+    def reset():
+        if not mixed:
+            A.copy_(A0)
+        Bw.copy_(B)
+
+    for _ in range(a.warmup):
+        reset()
+        plan.execute(A, Bw, info, None, iters)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({a.warmup} steps)")
+    # each step's inputs are restored outside the timed region (the FP64
+    # path factors A in place); events bracket the batched launch only
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    lmx.kernel_timing_reset()
+    lmx.kernel_timing(True)
+    for i in range(a.steps):
+        reset()
+        ev[i][0].record()
+        plan.execute(A, Bw, info, None, iters)
+        ev[i][1].record()
+    torch.cuda.synchronize()
+    lmx.kernel_timing(False)
+    lat = [e0.elapsed_time(e1) for e0, e1 in ev]
+    cnt, kms, kwork = lmx.kernel_timing_get("batch")
+    lmx.kernel_timing_reset()
+    ms_per_step = sum(lat) / len(lat)
+    flops = q["flops"]
+    infos = info.cpu().numpy()
+    its = iters.cpu().numpy()
+    eta = batch_backward_error(torch, probs, A0 if not mixed else A, Bw, B)
+    plan.close()
+    peak = 157.3 if mixed else 78.6
+    achieved = kwork / (kms * 1e-3) / 1e12 if kms > 0 else None
+    sps = len(shapes) / (ms_per_step * 1e-3)
+    result = {"metric": "FP64-accurate solves/sec & TFLOP/s vs plain FP64 GPU; p50/p99 solve latency",
+              "value": round(sps, 1), "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+              "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
+              "tflops": round(flops / (ms_per_step * 1e-3) / 1e12, 3),
+              "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+              "max_backward_error_over_neps": eta, "info_nonzero": int((infos != 0).sum()),
+              "config": {"workload": f"batched solves: {a.count} LU (A iid U[-1,1]) + {a.count} Cholesky "
+                                     "(A = G G^T/n + I), n log-uniform in [8,512], 1-8 RHS, "
+                                     + ("FP32 factor + FP64 refinement" if mixed else "FP64")
+                                     + ", one persistent launch (hipGraph)",
+                         "problems": len(shapes), "variant": q}}
+    if mixed:
+        result["refine"] = {"fell_back": int((its < 0).sum()), "mean_iterations": float(its[its >= 0].mean()),
+                            "max_iterations": int(its.max())}
+    result["roofline"] = {"bound": "mfma", "kernel": "batch_kernel", "achieved": round(achieved, 3) if achieved else None,
+                          "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4) if achieved else None,
+                          "traffic": None, "launches_per_step": cnt // max(a.steps, 1),
+                          "avg_launch_us": round(kms * 1e3 / max(cnt, 1), 2)}
+    if not a.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle) ...")
+        try:
+            result["cpu_baseline"] = cpu_baseline(a.workload, 0, 0, batch=(shapes, flops))
+            result["cpu_baseline"]["host_cores"] = os.cpu_count()
+        except Exception as e:
+            result["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(result), flush=True)
+
+
 # ------------------------------------------------------------------ main
 
 def main():
@@ -138,6 +310,8 @@ def main():
 
     lmx.init(0)
     wl = a.workload
+    if wl.startswith("batch"):
+        return bench_batch(a, np, torch, lmx)
     defaults = {"lu_mixed": (16384, 1), "chol_mixed": (16384, 1), "lu_fp64": (8192, 16),
                 "chol_fp64": (8192, 16), "gemm_fp64": (8192, 0)}
     n = a.n or defaults[wl][0]

@@ -185,6 +185,73 @@ int lmx_dposv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs,
                         int64_t ldb, double *dX, int64_t ldx,
                         lmx_refine_info *info);
 
+/* ---- batched solves of many small systems ------------------------------- */
+
+/* No reference counterpart (the reference solves one system per call, with
+ * luSolveP / choleskySolveP, src/.../Solve/LU.hs:265-304 and
+ * Solve/Cholesky.hs:241-273): these entries run thousands of independent
+ * solves of uneven sizes in one persistent launch, each to the reference's
+ * result within the stated tolerance. */
+#define LMX_BATCH_MAX_N 512
+#define LMX_BATCH_MAX_NRHS 8
+#define LMX_BATCH_LU 0
+#define LMX_BATCH_CHOLESKY 1
+/* plan flags */
+#define LMX_BATCH_FACTORS 1   /* FP64 LU: leave the factors in LAPACK layout
+                                 in A (else A holds them with rows in their
+                                 original order) */
+#define LMX_BATCH_GRAPH 2     /* capture the launch in a hipGraph and replay
+                                 it while the pool pointers are unchanged */
+#define LMX_BATCH_AUTOTUNE 4  /* time the kernel variants on synthetic
+                                 problems of the plan's shapes at creation
+                                 and keep the fastest */
+
+/* One problem of a batch.  A is n x n row-major (lda = n) at
+ * a_pool + a_offset; B is n x nrhs row-major (ldb = nrhs) at
+ * b_pool + b_offset and receives X; an LU problem's swap list (0-based,
+ * LAPACK order) goes to ipiv_pool + ipiv_offset when an ipiv pool is given.
+ * Cholesky reads only the lower triangle of A. */
+typedef struct lmx_batch_problem {
+    int32_t kind;       /* LMX_BATCH_LU or LMX_BATCH_CHOLESKY */
+    int32_t n;          /* 1 .. LMX_BATCH_MAX_N */
+    int32_t nrhs;       /* 1 .. LMX_BATCH_MAX_NRHS */
+    int32_t reserved;   /* ignored */
+    int64_t a_offset;   /* elements */
+    int64_t b_offset;   /* elements */
+    int64_t ipiv_offset;
+} lmx_batch_problem;
+
+typedef struct lmx_batch_plan lmx_batch_plan;
+
+/* Validates the problems against the pool sizes (in elements; the ipiv pool
+ * may be 0 = no pivots returned), orders them largest first, sizes the
+ * per-workgroup workspace and picks the kernel variant.  precision:
+ * LMX_PREC_F64 (FP64 factorisations), or LMX_PREC_F32 (FP32 factorisation
+ * on MFMA, FP64 refinement as LAPACK dsgesv, FP64 fallback; A is left
+ * unchanged unless a problem falls back). */
+int lmx_batch_plan_create(lmx_batch_plan **plan, const lmx_batch_problem *problems,
+                          int32_t count, int64_t a_pool_elems,
+                          int64_t b_pool_elems, int64_t ipiv_pool_elems,
+                          int32_t precision, int32_t flags);
+/* Runs every problem of the plan on device pools (stream-ordered).  info
+ * (count entries, device) gets 0, or the LAPACK info of a singular / not
+ * positive definite problem (its B is then unchanged); iterations (count
+ * entries, device, may be NULL) gets the refinement steps of a mixed
+ * solve, or -1 where it fell back to FP64. */
+int lmx_batch_plan_execute(lmx_batch_plan *plan, double *a_pool, double *b_pool,
+                           int32_t *ipiv_pool, int32_t *info,
+                           int32_t *iterations, void *stream);
+/* Re-runs the autotuner (synchronous). */
+int lmx_batch_plan_autotune(lmx_batch_plan *plan, void *stream);
+/* The chosen variant: block-column width, threads per workgroup, number of
+ * persistent workgroups, the batch's FP64-equivalent flops, and (4
+ * entries) the autotuner's time per variant in ms (0 if not tuned).
+ * Pointers may be NULL. */
+int lmx_batch_plan_query(const lmx_batch_plan *plan, int32_t *variant,
+                         int32_t *block_cols, int32_t *threads,
+                         int32_t *workgroups, double *flops, double *tuned_ms);
+int lmx_batch_plan_destroy(lmx_batch_plan *plan);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* When enabled, the library brackets every launch of the named kernel

@@ -2,15 +2,16 @@
 // LU solve, for row-major matrices.
 //
 // Panel kernel (getf2_panel_kernel): the m x kb panel (kb <= 32) is spread
-// over P cooperative workgroups of 256 threads, each owning 256*RPT
+// over P cooperative workgroups of PT = 512 threads, each owning PT*RPT
 // consecutive rows held in registers.  Per column j:
 //   1. each workgroup finds its candidate max |a_ij| over rows i >= j (ties:
 //      lowest row, NaN never chosen -- the rule of rawPivotSearch,
-//      src/.../Internal/Kernel.hs:1541-1556), wave shuffles + LDS;
-//   2. it publishes the candidate (value, row, full row data) in slot j%2
-//      and the owner of row j publishes row j; one grid barrier;
-//   3. every workgroup reduces the P candidates to the pivot p, the owners
-//      of rows j and p exchange them through the slots, and all rows i > j
+//      src/.../Internal/Kernel.hs:1541-1556): DPP wave reductions, then one
+//      LDS exchange between the waves;
+//   2. it publishes the candidate (value, row, full row data) in slot j%2,
+//      and workgroup 0 publishes row j, then raises its sequence number;
+//   3. every workgroup waits for all P sequence numbers, reduces the P
+//      candidates to the pivot p, swaps rows j and p, and all rows i > j
 //      get l = a_ij / a_pj (a division, as the reference) and the rank-1
 //      update of the remaining panel columns.
 // The current column always sits in register 0: after each column the
@@ -28,7 +29,9 @@
 #include <vector>
 
 #include "common.hpp"
+#include "device.hpp"
 #include "kernels.hpp"
+#include "sync.hpp"
 
 namespace lmx {
 
@@ -60,47 +63,6 @@ struct PanelSync {
     int diag_tag[2];
 };
 constexpr int SYNC_CLEAR_WORDS = MAXP + 16;
-
-// Wave-wide max / min on DPP lane permutations (no LDS round trip): quad
-// swaps, row rotations by 4 and 8, then the row broadcasts of lanes 15 and
-// 31; the full result lands in lane 63.  fmax ignores NaN, so a NaN never
-// wins the pivot search (as with rawPivotSearch's strict '>').
-template <int CTRL, int RM>
-__device__ inline int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, RM, 0xF, false);
-}
-template <int CTRL, int RM>
-__device__ inline float dpp_f(float v) {
-    return __int_as_float(dpp_i<CTRL, RM>(__float_as_int(v)));
-}
-template <int CTRL, int RM>
-__device__ inline double dpp_f(double v) {
-    return __hiloint2double(dpp_i<CTRL, RM>(__double2hiint(v)), dpp_i<CTRL, RM>(__double2loint(v)));
-}
-__device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
-__device__ inline double lane63(double v) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
-                            __builtin_amdgcn_readlane(__double2loint(v), 63));
-}
-template <class T>
-__device__ inline T wave_max(T v) {
-    v = fmax(v, dpp_f<0xB1, 0xF>(v));   // quad_perm 1,0,3,2
-    v = fmax(v, dpp_f<0x4E, 0xF>(v));   // quad_perm 2,3,0,1
-    v = fmax(v, dpp_f<0x124, 0xF>(v));  // row_ror 4
-    v = fmax(v, dpp_f<0x128, 0xF>(v));  // row_ror 8
-    v = fmax(v, dpp_f<0x142, 0xA>(v));  // row_bcast 15
-    v = fmax(v, dpp_f<0x143, 0xC>(v));  // row_bcast 31
-    return lane63(v);
-}
-__device__ inline int wave_min(int v) {
-    v = min(v, dpp_i<0xB1, 0xF>(v));
-    v = min(v, dpp_i<0x4E, 0xF>(v));
-    v = min(v, dpp_i<0x124, 0xF>(v));
-    v = min(v, dpp_i<0x128, 0xF>(v));
-    v = min(v, dpp_i<0x142, 0xA>(v));
-    v = min(v, dpp_i<0x143, 0xC>(v));
-    return __builtin_amdgcn_readlane(v, 63);
-}
 
 template <class T>
 __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
@@ -182,50 +144,52 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
 
         int p, ptag;
         if (P > 1) {
-            // 3. publish: plain stores by wave 0, one release, then the sequence number
+            // 3. publish: device-coherent stores by wave 0, drained, then the
+            //    sequence number
             if (wave == 0) {
-                if (lane < KB) sy->cand_data[par][w][lane] = s_cand[lane];
-                if (lane == KB) sy->cand_val[par][w] = best;
-                if (lane == KB + 1) sy->cand_row[par][w] = brow;
-                if (lane == KB + 2) sy->cand_tag[par][w] = s_ctag;
+                if (lane < KB) xstore(&sy->cand_data[par][w][lane], s_cand[lane]);
+                if (lane == KB) xstore(&sy->cand_val[par][w], best);
+                if (lane == KB + 1) xstore(&sy->cand_row[par][w], brow);
+                if (lane == KB + 2) xstore(&sy->cand_tag[par][w], s_ctag);
                 if (w == 0) {
-                    if (lane < KB) sy->diag_data[par][lane] = s_drow[lane];
-                    if (lane == KB + 3) sy->diag_tag[par] = s_dtag;
+                    if (lane < KB) xstore(&sy->diag_data[par][lane], s_drow[lane]);
+                    if (lane == KB + 3) xstore(&sy->diag_tag[par], s_dtag);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (lane == 0)
-                    __hip_atomic_store(&sy->seq[w], base + (unsigned)j + 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                xdrain();
+                if (lane == 0) xstore(&sy->seq[w], base + (unsigned)j + 1u);
             }
             // 4. wait until every workgroup has published column j
             if (t < P && !failed) {
                 const unsigned want = base + (unsigned)j + 1u;
                 long spins = 0;
-                while (__hip_atomic_load(&sy->seq[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-                    if (__hip_atomic_load(&sy->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                while (xload(&sy->seq[t]) < want) {
+                    if (xload(&sy->error) != 0u) {
                         failed = true;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > SPIN_LIMIT) {
-                        __hip_atomic_store(&sy->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        xstore(&sy->error, 1u);
                         failed = true;
                         break;
                     }
                 }
             }
             __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            // 5. fetch every candidate record in one round of plain loads
+            // 5. fetch every candidate record with device-coherent loads
             for (int e = t; e < P; e += PT) {
-                s_val[e] = sy->cand_val[par][e];
-                s_row[e] = sy->cand_row[par][e];
-                s_tag[e] = sy->cand_tag[par][e];
+                s_val[e] = xload(&sy->cand_val[par][e]);
+                s_row[e] = xload(&sy->cand_row[par][e]);
+                s_tag[e] = xload(&sy->cand_tag[par][e]);
             }
-            for (int e = t; e < P * KB; e += PT) s_all[e / KB][e % KB] = sy->cand_data[par][e / KB][e % KB];
+#pragma unroll
+            for (int i = 0; i < MAXP * KB / PT; ++i) {
+                const int e = t + i * PT;
+                if (e < P * KB) s_all[e / KB][e % KB] = xload(&sy->cand_data[par][e / KB][e % KB]);
+            }
             if (w != 0) {
-                if (t < KB) s_drow[t] = sy->diag_data[par][t];
-                if (t == 0) s_dtag = sy->diag_tag[par];
+                if (t < KB) s_drow[t] = xload(&sy->diag_data[par][t]);
+                if (t == 0) s_dtag = xload(&sy->diag_tag[par]);
             }
             __syncthreads();
             // 6. the global pivot, from LDS (same rule as within a workgroup)

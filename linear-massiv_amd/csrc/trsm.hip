@@ -15,7 +15,9 @@
 //     subtracts M_bj x_j on MFMA, and finally writes x_b = W_b acc and raises
 //     its own flag -- no host round trip and no per-block launch.
 #include "common.hpp"
+#include "device.hpp"
 #include "kernels.hpp"
+#include "sync.hpp"
 
 namespace lmx {
 
@@ -23,27 +25,6 @@ namespace {
 
 constexpr int TB = 64;
 constexpr long CHAIN_SPIN_LIMIT = 1L << 24;
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-template <class T>
-struct V4;
-template <>
-struct V4<double> {
-    typedef d4 type;
-};
-template <>
-struct V4<float> {
-    typedef f4 type;
-};
-__device__ inline d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-__device__ inline f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-// Row, inside a 16 x 16 MFMA block, of accumulator value r of lane group lk
-// (f64: lk + 4r, f32: 4lk + r; measured on gfx950).
-template <class T>
-__device__ inline int acc_row(int lk, int r) {
-    return sizeof(T) == 8 ? lk + 4 * r : 4 * lk + r;
-}
 
 // Element (r, c) of M = op(T) (MT = transpose), restricted to M's triangle.
 template <class T, bool MT, bool MLOWER>
@@ -162,15 +143,6 @@ __global__ __launch_bounds__(64) void trsm_leaf_kernel(int nb, int nvec, const T
 
 // ---------------------------------------------------------------- chain kernel
 
-template <class T>
-__device__ inline T ld_agent(const T *p) {
-    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
-__device__ inline void st_agent(T *p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // The 64 x 64 block of M at (r0, c0) (zero outside the matrix), fetched
 // into 16 registers per thread so all loads are in flight together, and
 // stored to LDS separately.
@@ -192,11 +164,10 @@ __device__ inline void store_mblock(T (*Ms)[LD], const T (&reg)[16], int t) {
     }
 }
 
-// Visibility between workgroups: a block's solution is written with plain
-// stores, then every wave of the writer executes an agent-scope release
-// fence before thread 0 publishes the flag with a release store; the
-// reader's thread 0 acquires the flag, and after the barrier every wave
-// executes an agent-scope acquire fence before its plain loads.
+// Visibility between workgroups: a block's solution is written to B with
+// device-coherent stores (xstore), every wave drains them, and after a
+// barrier thread 0 raises the flag; the reader's thread 0 spins on the flag,
+// and after the barrier the block is read with device-coherent loads.
 template <class T, bool MT, bool MLOWER>
 __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int nrhs, const T *__restrict__ A, int64_t lda,
                                                          const T *__restrict__ W, T *B, int64_t ldb,
@@ -241,21 +212,24 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int nrhs, const 
         const int cur = d & 1;
         if (t == 0 && !failed) {
             long spins = 0;
-            while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            while (xload(&flags[j]) == 0) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > CHAIN_SPIN_LIMIT) {
-                    st_agent(err, 1);
+                    xstore(err, 1);
                     failed = true;
                     break;
                 }
             }
         }
         __syncthreads();  // Ms[cur] loaded, flag j seen, Xs free
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        for (int e = t; e < TB * nrhs; e += 256) {
-            int r = e / nrhs, c = e - r * nrhs;
-            int64_t gr = c0 + r;
-            Xs[r][c] = gr < n ? B[gr * ldb + c] : T(0);
+#pragma unroll
+        for (int i = 0; i < TB * TB / 256; ++i) {
+            const int e = t + 256 * i;
+            if (e < TB * nrhs) {
+                int r = e / nrhs, c = e - r * nrhs;
+                int64_t gr = c0 + r;
+                Xs[r][c] = gr < n ? xload(&B[gr * ldb + c]) : T(0);
+            }
         }
         // the next block's loads fly while this block is used; they land in
         // the other LDS buffer after the MFMAs
@@ -293,11 +267,11 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int nrhs, const 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             int gi = r0 + wave * 16 + acc_row<T>(lk, r), gc = cb * 16 + lr;
-            if (cb < ncb && gi < n && gc < nrhs) B[(int64_t)gi * ldb + gc] = out[cb][r];
+            if (cb < ncb && gi < n && gc < nrhs) xstore(&B[(int64_t)gi * ldb + gc], out[cb][r]);
         }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    xdrain();
     __syncthreads();
-    if (t == 0) __hip_atomic_store(&flags[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 0) xstore(&flags[b], 1);
 }
 
 __global__ void zero_i32_kernel(int *p, int n) {

@@ -59,8 +59,13 @@ EXPORTED = [
     "lmx_dpotrf", "lmx_dpotrf_dev", "lmx_dpotrs", "lmx_dpotrs_dev", "lmx_dposv", "lmx_dposv_dev",
     "lmx_dtrsm", "lmx_dtrsm_dev",
     "lmx_dgesv_mixed", "lmx_dgesv_mixed_dev", "lmx_dposv_mixed", "lmx_dposv_mixed_dev",
+    "lmx_batch_plan_create", "lmx_batch_plan_execute", "lmx_batch_plan_autotune", "lmx_batch_plan_query",
+    "lmx_batch_plan_destroy",
     "lmx_kernel_timing", "lmx_kernel_timing_reset", "lmx_kernel_timing_get",
 ]
+BATCH_MAX_N, BATCH_MAX_NRHS = 512, 8
+BATCH_LU, BATCH_CHOLESKY = 0, 1
+BATCH_FACTORS, BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2, 4
 
 
 class LmxError(RuntimeError):
@@ -75,7 +80,15 @@ class RefineInfo(ctypes.Structure):
                 ("backward_error", ctypes.c_double)]
 
 
+class BatchProblem(ctypes.Structure):
+    """lmx_batch_problem (include/lmx.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("n", ctypes.c_int32), ("nrhs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("a_offset", ctypes.c_int64), ("b_offset", ctypes.c_int64),
+                ("ipiv_offset", ctypes.c_int64)]
+
+
 _i64, _dbl, _flt, _vp, _chr = ctypes.c_int64, ctypes.c_double, ctypes.c_float, ctypes.c_void_p, ctypes.c_char
+_i32 = ctypes.c_int32
 _pRI = ctypes.POINTER(RefineInfo)
 
 _sigs = {
@@ -102,6 +115,13 @@ _sigs = {
     "lmx_dgesv_mixed_dev": [_vp, ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _pRI],
     "lmx_dposv_mixed": [ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _pRI],
     "lmx_dposv_mixed_dev": [_vp, ctypes.c_int, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _pRI],
+    "lmx_batch_plan_create": [ctypes.POINTER(_vp), ctypes.POINTER(BatchProblem), _i32, _i64, _i64, _i64, _i32,
+                              _i32],
+    "lmx_batch_plan_execute": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "lmx_batch_plan_autotune": [_vp, _vp],
+    "lmx_batch_plan_query": [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                             ctypes.POINTER(_i32), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)],
+    "lmx_batch_plan_destroy": [_vp],
     "lmx_kernel_timing": [ctypes.c_int], "lmx_kernel_timing_reset": [],
     "lmx_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)],
 }
@@ -422,6 +442,92 @@ def posv_mixed_dev(A, B, X, precision=PREC_F32, stream=None):
     _check(_lib.lmx_dposv_mixed_dev(_stream(stream), precision, A.shape[0], B.shape[1], _tp(A), A.stride(0),
                                     _tp(B), B.stride(0), _tp(X), X.stride(0), ctypes.byref(ri)), allow_info=True)
     return ri
+
+
+# ---------------------------------------------------------------- batched solves
+
+def batch_layout(shapes):
+    """Packs problems of the given (kind, n, nrhs) shapes back to back:
+    -> (list of BatchProblem, a_pool elements, b_pool elements, ipiv elements)."""
+    probs, ao, bo, po = [], 0, 0, 0
+    for kind, n, nrhs in shapes:
+        probs.append(BatchProblem(kind, n, nrhs, 0, ao, bo, po))
+        ao += n * n
+        bo += n * nrhs
+        po += n
+    return probs, ao, bo, po
+
+
+class BatchPlan:
+    """A plan for one batch of independent LU / Cholesky solves
+    (lmx_batch_plan_*); execute() runs them on torch tensors in HBM."""
+
+    def __init__(self, problems, a_elems, b_elems, ipiv_elems=0, precision=PREC_F64, flags=0):
+        arr = (BatchProblem * max(len(problems), 1))(*problems)
+        h = _vp()
+        _check(_lib.lmx_batch_plan_create(ctypes.byref(h), arr, len(problems), a_elems, b_elems, ipiv_elems,
+                                          precision, flags))
+        self._h = h
+        self.count = len(problems)
+
+    def execute(self, a_pool, b_pool, info, ipiv_pool=None, iterations=None, stream=None):
+        _check(_lib.lmx_batch_plan_execute(self._h, _tp(a_pool), _tp(b_pool),
+                                           _tp(ipiv_pool) if ipiv_pool is not None else None, _tp(info),
+                                           _tp(iterations) if iterations is not None else None,
+                                           _stream(stream)))
+
+    def autotune(self, stream=None):
+        _check(_lib.lmx_batch_plan_autotune(self._h, _stream(stream)))
+
+    def query(self):
+        v, nb, nt, wg = _i32(), _i32(), _i32(), _i32()
+        fl = _dbl()
+        ms = (_dbl * 4)()
+        _check(_lib.lmx_batch_plan_query(self._h, ctypes.byref(v), ctypes.byref(nb), ctypes.byref(nt),
+                                         ctypes.byref(wg), ctypes.byref(fl), ms))
+        return {"variant": v.value, "block_cols": nb.value, "threads": nt.value, "workgroups": wg.value,
+                "flops": fl.value, "tuned_ms": list(ms)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lmx_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batched(problems, precision=PREC_F64, flags=0, return_factors=False):
+    """Solves a list of (kind, A, B) numpy problems in one batched launch.
+    -> (list of X, info array, iterations array[, list of (A_out, ipiv)])."""
+    import torch
+    shapes = [(k, A.shape[0], B.shape[1]) for k, A, B in problems]
+    probs, na, nb, npv = batch_layout(shapes)
+    a = np.concatenate([np.ascontiguousarray(A, dtype=np.float64).ravel() for _, A, _ in problems])
+    b = np.concatenate([np.ascontiguousarray(B, dtype=np.float64).ravel() for _, _, B in problems])
+    dA = torch.tensor(a, device="cuda")
+    dB = torch.tensor(b, device="cuda")
+    dP = torch.zeros(max(npv, 1), dtype=torch.int32, device="cuda")
+    info = torch.full((len(problems),), -7, dtype=torch.int32, device="cuda")
+    iters = torch.full((len(problems),), -7, dtype=torch.int32, device="cuda")
+    plan = BatchPlan(probs, na, nb, npv, precision, flags)
+    try:
+        plan.execute(dA, dB, info, dP, iters)
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    bh = dB.cpu().numpy()
+    X = [bh[p.b_offset:p.b_offset + p.n * p.nrhs].reshape(p.n, p.nrhs) for p in probs]
+    out = (X, info.cpu().numpy(), iters.cpu().numpy())
+    if return_factors:
+        ah, ph = dA.cpu().numpy(), dP.cpu().numpy()
+        fac = [(ah[p.a_offset:p.a_offset + p.n * p.n].reshape(p.n, p.n), ph[p.ipiv_offset:p.ipiv_offset + p.n])
+               for p in probs]
+        out = out + (fac,)
+    return out
 
 
 # ---------------------------------------------------------------- kernel timing

@@ -67,3 +67,36 @@ def test_zero_sized_calls_are_no_ops():
     assert L.lmx_dgetrf(0, None, 1, None) == 0
     assert L.lmx_dpotrf(0, None, 1) == 0
     assert L.lmx_dtrsm(b"L", b"N", b"N", 0, 3, None, 1, None, 3) == 0
+
+
+@pytest.mark.parametrize("shape,pools,prec,msg", [
+    ((0, 0, 1), None, 0, "n must be"),
+    ((0, 513, 1), None, 0, "n must be"),
+    ((0, 8, 0), None, 0, "nrhs must be"),
+    ((0, 8, 9), None, 0, "nrhs must be"),
+    ((2, 8, 1), None, 0, "kind"),
+    ((0, 8, 1), (63, 8, 8), 0, "A lies outside"),
+    ((1, 8, 2), (64, 15, 0), 0, "B lies outside"),
+    ((0, 8, 2), (64, 16, 7), 0, "ipiv lies outside"),
+    ((0, 8, 1), None, 2, "LMX_PREC_F64 and LMX_PREC_F32"),
+])
+def test_batch_plan_validation(shape, pools, prec, msg):
+    # every check runs before any device call
+    import lmx
+    probs, na, nb, npv = lmx.batch_layout([shape])
+    if pools is not None:
+        na, nb, npv = pools
+    with pytest.raises(lmx.LmxError) as e:
+        lmx.BatchPlan(probs, na, nb, npv, precision=prec)
+    assert e.value.code == lmx.LMX_ERR_ARG
+    assert msg in str(e.value)
+
+
+def test_batch_layout_packs_back_to_back():
+    import lmx
+    probs, na, nb, npv = lmx.batch_layout([(0, 3, 2), (1, 5, 1), (0, 2, 8)])
+    assert [p.a_offset for p in probs] == [0, 9, 34]
+    assert [p.b_offset for p in probs] == [0, 6, 11]
+    assert [p.ipiv_offset for p in probs] == [0, 3, 8]
+    assert (na, nb, npv) == (38, 27, 10)
+    assert ctypes.sizeof(lmx.BatchProblem) == 40
