@@ -14,7 +14,10 @@ Prints ONE JSON line.  Other workloads (--workload): lu_fp64 / chol_fp64
 workloads batch_mixed / batch_fp64: 20,000 LU (A iid U[-1,1]) plus 20,000
 Cholesky (A = G G^T/n + I) independent solves, n log-uniform in [8, 512],
 1-8 right-hand sides each, in one persistent launch (one step = the whole
-batch).
+batch); and serve: Poisson arrivals of solve requests (70% Cholesky, 30%
+LU, n log-uniform in [16, 4096], 1-32 RHS, 30% reusing a resident factor)
+through the solve server, FP32 factor + FP64 refinement, at rates swept
+up to ~80% device utilisation; reports p50/p99 latency and throughput.
 """
 import argparse
 import json
@@ -39,7 +42,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="lu_mixed",
                    choices=["lu_mixed", "chol_mixed", "lu_fp64", "chol_fp64", "gemm_fp64", "batch_mixed",
-                            "batch_fp64"])
+                            "batch_fp64", "serve"])
+    p.add_argument("--requests", type=int, default=400, help="serve: requests per rate point")
     p.add_argument("--count", type=int, default=20000, help="problems per family (batched workloads)")
     p.add_argument("--autotune", action="store_true", help="batched workloads: autotune the kernel variant")
     p.add_argument("--n", type=int, default=None)
@@ -296,6 +300,152 @@ def bench_batch(a, np, torch, lmx):
     print(json.dumps(result), flush=True)
 
 
+# ------------------------------------------------------------------ serving
+
+def serve_library(torch, np, count, seed):
+    """Matrices the clients send: 70% SPD (G G^T/n + I), 30% U[-1,1], n
+    log-uniform in [16, 4096], generated on the device, kept in pinned host
+    memory as a client's buffers would be."""
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    lib = []
+    for i in range(count):
+        kind = 1 if rng.random() < 0.7 else 0
+        n = int(round(np.exp(rng.uniform(np.log(16), np.log(4096)))))
+        if kind:
+            G = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+            A = G @ G.T / n + torch.eye(n, dtype=torch.float64, device="cuda")
+        else:
+            A = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        h = torch.empty((n, n), dtype=torch.float64, pin_memory=True)
+        h.copy_(A)
+        lib.append((kind, n, h.numpy(), h))
+    torch.cuda.synchronize()
+    return lib
+
+
+def serve_run(lmx, np, srv, lib, nreq, rate, seed, rhs_pool):
+    """Open loop (Poisson arrivals at `rate` req/s; rate <= 0: all at once).
+    -> per-request records and the wall time."""
+    import threading
+    rng = np.random.default_rng(seed)
+    live = []            # resident factors: (handle, kind, n)
+    lock = threading.Lock()
+    recs = []
+    pending = []
+    cv = threading.Condition()
+    done_flag = [False]
+
+    def collector():
+        while True:
+            with cv:
+                while not pending and not done_flag[0]:
+                    cv.wait()
+                if not pending and done_flag[0]:
+                    return
+                rid, meta = pending.pop(0)
+            r = srv.wait(rid)
+            recs.append((meta, r.rc, r.latency_ms, r.queue_ms, r.iterations, r.fell_back))
+            if r.factor:
+                with lock:
+                    live.append((r.factor, meta[1], meta[2]))
+                    while len(live) > 12:
+                        srv.release(live.pop(0)[0])
+
+    th = threading.Thread(target=collector)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for i in range(nreq):
+        if rate > 0:
+            t_next += rng.exponential(1.0 / rate)
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        nrhs = int(rng.integers(1, 33))
+        reuse = None
+        if rng.random() < 0.3:
+            with lock:
+                if live:
+                    reuse = live[int(rng.integers(0, len(live)))]
+        if reuse:
+            fh, kind, n = reuse
+            B = np.ascontiguousarray(rhs_pool[:n * nrhs].reshape(n, nrhs))
+            X = np.empty_like(B)
+            rid = srv.submit(kind, B, X, factor=fh)
+            meta = ("reuse", kind, n, nrhs)
+        else:
+            kind, n, A, _ = lib[int(rng.integers(0, len(lib)))]
+            B = np.ascontiguousarray(rhs_pool[:n * nrhs].reshape(n, nrhs))
+            X = np.empty_like(B)
+            keep = rng.random() < 0.5
+            rid = srv.submit(kind, B, X, A=A, keep=keep)
+            meta = ("new", kind, n, nrhs)
+        with cv:
+            pending.append((rid, meta))
+            cv.notify()
+    with cv:
+        done_flag[0] = True
+        cv.notify()
+    th.join()
+    wall = time.perf_counter() - t0
+    for fh, _, _ in live:
+        srv.release(fh)
+    return recs, wall
+
+
+def serve_flops(meta):
+    typ, kind, n, nrhs = meta
+    solve = 2.0 * n * n * nrhs
+    if typ == "reuse":
+        return solve
+    return (n ** 3 / 3.0 if kind == 1 else 2.0 * n ** 3 / 3.0) + solve
+
+
+def bench_serve(a, np, torch, lmx):
+    t0 = time.perf_counter()
+    lib = serve_library(torch, np, 48, a.seed)
+    rhs_pool = np.random.default_rng(a.seed + 1).uniform(-1, 1, 4096 * 32)
+    log(f"[bench] serve: library of {len(lib)} matrices in {time.perf_counter() - t0:.1f} s")
+    srv = lmx.Server(budget_bytes=32 << 30, slots=4, small_n=256)
+    try:
+        # warm-up, then the closed-loop capacity (all requests at once)
+        serve_run(lmx, np, srv, lib, 40, 0, a.seed + 2, rhs_pool)
+        srv.stats(reset=True)
+        recs, wall = serve_run(lmx, np, srv, lib, a.requests, 0, a.seed + 3, rhs_pool)
+        st = srv.stats(reset=True)
+        capacity = len(recs) / wall
+        log(f"[bench] capacity {capacity:.1f} req/s (busy {st['busy_ms'] / st['window_ms']:.2f})")
+        points = []
+        for frac in (0.5, 0.8, 0.9):
+            rate = frac * capacity
+            recs, wall = serve_run(lmx, np, srv, lib, a.requests, rate, a.seed + 4 + int(frac * 10), rhs_pool)
+            st = srv.stats(reset=True)
+            lat = np.array([r[2] for r in recs])
+            flops = sum(serve_flops(r[0]) for r in recs)
+            pt = {"offered_req_per_s": round(rate, 1), "req_per_s": round(len(recs) / wall, 1),
+                  "tflops": round(flops / wall / 1e12, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                  "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                  "utilisation": round(st["busy_ms"] / st["window_ms"], 3), "failed": int(sum(r[1] != 0 for r in recs)),
+                  "fell_back": int(sum(r[5] for r in recs)), "peak_hbm_gib": round(st["peak_bytes"] / 2**30, 2)}
+            log(f"[bench] rate {rate:.1f}: {pt}")
+            points.append(pt)
+    finally:
+        srv.close()
+    best = min(points, key=lambda p: abs(p["utilisation"] - 0.8))
+    result = {"metric": "FP64-accurate solves/sec & TFLOP/s vs plain FP64 GPU; p50/p99 solve latency",
+              "value": best["req_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": a.requests, "warmup": 40,
+              "ms_per_step": round(1e3 / best["req_per_s"], 3), "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+              "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "tflops": best["tflops"],
+              "utilisation": best["utilisation"], "capacity_req_per_s": round(capacity, 1), "sweep": points,
+              "config": {"workload": "serving: Poisson arrivals, 70% Cholesky / 30% LU, n log-uniform in [16,4096], "
+                                     "1-32 RHS, 30% reuse a resident factor; FP32 factor + FP64 refinement; "
+                                     "HBM budget 32 GiB, 4 slots, n <= 256 batched",
+                         "requests_per_point": a.requests}}
+    print(json.dumps(result), flush=True)
+
+
 # ------------------------------------------------------------------ main
 
 def main():
@@ -312,6 +462,8 @@ def main():
     wl = a.workload
     if wl.startswith("batch"):
         return bench_batch(a, np, torch, lmx)
+    if wl == "serve":
+        return bench_serve(a, np, torch, lmx)
     defaults = {"lu_mixed": (16384, 1), "chol_mixed": (16384, 1), "lu_fp64": (8192, 16),
                 "chol_fp64": (8192, 16), "gemm_fp64": (8192, 0)}
     n = a.n or defaults[wl][0]

@@ -252,6 +252,71 @@ int lmx_batch_plan_query(const lmx_batch_plan *plan, int32_t *variant,
                          int32_t *workgroups, double *flops, double *tuned_ms);
 int lmx_batch_plan_destroy(lmx_batch_plan *plan);
 
+/* ---- solve server ------------------------------------------------------- */
+
+/* No reference counterpart: a request loop over the mixed-precision solvers
+ * for callers that issue many independent solves (the reference's
+ * luSolveP / choleskySolveP, one call each).  Requests are admitted as
+ * worker slots and HBM (within the budget) free up, run concurrently on
+ * separate streams, small ones grouped into one batched launch; a factor
+ * can be kept resident and reused for new right-hand sides. */
+typedef struct lmx_server_config {
+    int64_t budget_bytes; /* HBM the server may use; 0 = 80% of free memory */
+    int32_t slots;        /* concurrent large requests (1..16; 0 = 4) */
+    int32_t small_n;      /* requests with n <= small_n (<= 512) and at most
+                             8 right-hand sides go through the batched kernel
+                             in groups; 0 = never */
+    int32_t small_max;    /* most requests in one batched launch (0 = 256) */
+    int32_t reserved;
+    double aging_ms;      /* a queued request older than this is not
+                             overtaken by smaller later ones (0 = 100 ms) */
+} lmx_server_config;
+
+#define LMX_REQ_KEEP_FACTOR 1 /* keep the factor resident; its handle is
+                                 returned in lmx_request_result.factor */
+
+typedef struct lmx_request_result {
+    int32_t status;        /* 1 = done, 0 = still queued or running */
+    int32_t rc;            /* LMX_OK, LAPACK info > 0, or an error < 0 */
+    int32_t iterations;    /* refinement steps */
+    int32_t fell_back;     /* 1 if it was solved by an FP64 factorisation */
+    double backward_error; /* from the refinement (large requests) */
+    int64_t factor;        /* resident factor handle (KEEP_FACTOR), else 0 */
+    double queue_ms;       /* submit -> admitted */
+    double run_ms;         /* admitted -> done */
+    double latency_ms;     /* submit -> done */
+} lmx_request_result;
+
+typedef struct lmx_server_counters {
+    int64_t completed, failed, queued, inflight;
+    int64_t bytes_in_use, peak_bytes, budget_bytes, resident_factors;
+    double busy_ms;   /* union of the requests' device intervals */
+    double window_ms; /* wall time of the same window */
+} lmx_server_counters;
+
+typedef struct lmx_server lmx_server;
+
+int lmx_server_create(lmx_server **server, const lmx_server_config *config);
+/* Queues one solve A X = B (kind LMX_BATCH_LU or LMX_BATCH_CHOLESKY, host
+ * row-major buffers that must stay valid until the request is done).
+ * Either A (with lda) or a resident factor handle of the same kind and n is
+ * given.  Fails with LMX_ERR_BUDGET if the request can never fit. */
+int lmx_server_submit(lmx_server *server, int32_t kind, int64_t n, int64_t nrhs,
+                      const double *A, int64_t lda, int64_t factor,
+                      const double *B, int64_t ldb, double *X, int64_t ldx,
+                      int32_t flags, int64_t *request_id);
+/* Blocks until the request is done, fills the result and forgets it. */
+int lmx_server_wait(lmx_server *server, int64_t request_id, lmx_request_result *result);
+/* LMX_OK and the result if done (then forgets it), 1 if not yet. */
+int lmx_server_poll(lmx_server *server, int64_t request_id, lmx_request_result *result);
+/* Frees a resident factor once the requests using it are done. */
+int lmx_server_factor_release(lmx_server *server, int64_t factor);
+/* Counters and the device-busy fraction (busy_ms / window_ms) since the
+ * last reset; reset != 0 starts a new window. */
+int lmx_server_stats(lmx_server *server, lmx_server_counters *stats, int32_t reset);
+/* Finishes running requests, fails queued ones, frees everything. */
+int lmx_server_destroy(lmx_server *server);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* When enabled, the library brackets every launch of the named kernel

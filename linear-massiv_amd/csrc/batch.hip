@@ -946,6 +946,7 @@ struct lmx_batch_plan {
     int32_t *g_ipiv = nullptr, *g_info = nullptr, *g_iters = nullptr;
     int g_variant = -1;
     double tuned_ms[lmx::NVARIANTS] = {0, 0, 0, 0};
+    size_t probs_cap = 0;
 };
 
 namespace lmx {
@@ -1121,6 +1122,78 @@ void autotune(lmx_batch_plan *pl, hipStream_t s) {
 }
 
 }  // namespace
+
+void batch_replan(lmx_batch_plan *pl, const lmx_batch_problem *problems, int32_t count, int64_t a_pool_elems,
+                  int64_t b_pool_elems, int64_t ipiv_pool_elems, hipStream_t s) {
+    if (count < 0) fail_arg("count < 0");
+    if (count > 0 && !problems) fail_arg("problems is NULL");
+    std::vector<lmx_batch_problem> sorted(problems, problems + count);
+    int nmax = 0;
+    double flops = 0;
+    for (int i = 0; i < count; ++i) {
+        lmx_batch_problem &p = sorted[i];
+        const std::string at = "problem " + std::to_string(i) + ": ";
+        if (p.kind != LMX_BATCH_LU && p.kind != LMX_BATCH_CHOLESKY) fail_arg(at + "kind must be LU or Cholesky");
+        if (p.n < 1 || p.n > BN) fail_arg(at + "n must be in [1, " + std::to_string(BN) + "]");
+        if (p.nrhs < 1 || p.nrhs > BR) fail_arg(at + "nrhs must be in [1, " + std::to_string(BR) + "]");
+        const int64_t n = p.n;
+        if (p.a_offset < 0 || p.a_offset + n * n > a_pool_elems) fail_arg(at + "A lies outside the A pool");
+        if (p.b_offset < 0 || p.b_offset + n * p.nrhs > b_pool_elems) fail_arg(at + "B lies outside the B pool");
+        if (p.kind == LMX_BATCH_LU && ipiv_pool_elems > 0 &&
+            (p.ipiv_offset < 0 || p.ipiv_offset + n > ipiv_pool_elems))
+            fail_arg(at + "ipiv lies outside the ipiv pool");
+        p.reserved = i;
+        nmax = std::max(nmax, p.n);
+        flops += problem_work(p);
+    }
+    // largest first (longest-processing-time order)
+    std::stable_sort(sorted.begin(), sorted.end(), [](const lmx_batch_problem &x, const lmx_batch_problem &y) {
+        return problem_work(x) > problem_work(y);
+    });
+    ctx();
+    pl->count = count;
+    pl->a_elems = a_pool_elems;
+    pl->b_elems = b_pool_elems;
+    pl->ipiv_elems = ipiv_pool_elems;
+    pl->nmax = nmax;
+    pl->flops = flops;
+    pl->sorted.swap(sorted);
+    if ((size_t)count > pl->probs_cap) {
+        if (pl->d_probs) LMX_HIP(hipFree(pl->d_probs));
+        pl->d_probs = nullptr;
+        pl->probs_cap = 0;
+        LMX_HIP(hipMalloc(&pl->d_probs, sizeof(lmx_batch_problem) * (size_t)count));
+        pl->probs_cap = (size_t)count;
+    }
+    if (count > 0) {
+        if (s) {
+            LMX_HIP(hipMemcpyAsync(pl->d_probs, pl->sorted.data(), sizeof(lmx_batch_problem) * (size_t)count,
+                                   hipMemcpyHostToDevice, s));
+        } else {
+            LMX_HIP(hipMemcpy(pl->d_probs, pl->sorted.data(), sizeof(lmx_batch_problem) * (size_t)count,
+                              hipMemcpyHostToDevice));
+        }
+    }
+    if (!pl->d_counter) LMX_HIP(hipMalloc(&pl->d_counter, 256));
+    pl->args.probs = pl->d_probs;
+    pl->args.count = count;
+    pl->args.counter = pl->d_counter;
+    drop_graph(pl);
+    if (count > 0) configure(pl, pl->variant);
+}
+
+size_t batch_slot_bytes(int nmax, int precision, int flags) {
+    const int nb = VARIANTS[0].nb;
+    const int64_t nmax_pad = cdiv(std::max(nmax, 1), nb) * nb;
+    const bool mixed = precision == LMX_PREC_F32;
+    auto r = [](int64_t b) { return (b + 255) / 256 * 256; };
+    int64_t b = r(2LL * nmax_pad * 32 * 8) + (mixed ? r(2LL * nmax_pad * 32 * 4) + r((int64_t)nmax * BR * 8) +
+                                                          r((int64_t)nmax * nmax * 4)
+                                                    : 0) +
+                ((flags & LMX_BATCH_FACTORS) ? r((int64_t)nmax * nmax * 8) : 0);
+    return (size_t)b;
+}
+
 }  // namespace lmx
 
 using namespace lmx;
@@ -1133,56 +1206,17 @@ int lmx_batch_plan_create(lmx_batch_plan **plan, const lmx_batch_problem *proble
     return guarded([&]() -> int {
         if (!plan) fail_arg("plan is NULL");
         *plan = nullptr;
-        if (count < 0) fail_arg("count < 0");
-        if (count > 0 && !problems) fail_arg("problems is NULL");
         if (precision != LMX_PREC_F64 && precision != LMX_PREC_F32)
             fail_arg("batched solves support LMX_PREC_F64 and LMX_PREC_F32 (FP32 factor, FP64 refinement)");
         if (flags & ~(LMX_BATCH_FACTORS | LMX_BATCH_GRAPH | LMX_BATCH_AUTOTUNE)) fail_arg("unknown flag bits");
         std::unique_ptr<lmx_batch_plan> pl(new lmx_batch_plan());
-        pl->count = count;
         pl->precision = precision;
         pl->flags = flags;
-        pl->a_elems = a_pool_elems;
-        pl->b_elems = b_pool_elems;
-        pl->ipiv_elems = ipiv_pool_elems;
-        pl->sorted.assign(problems, problems + count);
-        for (int i = 0; i < count; ++i) {
-            lmx_batch_problem &p = pl->sorted[i];
-            const std::string at = "problem " + std::to_string(i) + ": ";
-            if (p.kind != LMX_BATCH_LU && p.kind != LMX_BATCH_CHOLESKY) fail_arg(at + "kind must be LU or Cholesky");
-            if (p.n < 1 || p.n > BN) fail_arg(at + "n must be in [1, " + std::to_string(BN) + "]");
-            if (p.nrhs < 1 || p.nrhs > BR) fail_arg(at + "nrhs must be in [1, " + std::to_string(BR) + "]");
-            const int64_t n = p.n;
-            if (p.a_offset < 0 || p.a_offset + n * n > a_pool_elems) fail_arg(at + "A lies outside the A pool");
-            if (p.b_offset < 0 || p.b_offset + n * p.nrhs > b_pool_elems) fail_arg(at + "B lies outside the B pool");
-            if (p.kind == LMX_BATCH_LU && ipiv_pool_elems > 0 &&
-                (p.ipiv_offset < 0 || p.ipiv_offset + n > ipiv_pool_elems))
-                fail_arg(at + "ipiv lies outside the ipiv pool");
-            p.reserved = i;
-            pl->nmax = std::max(pl->nmax, p.n);
-            pl->flops += problem_work(p);
-        }
-        // largest first (longest-processing-time order)
-        std::stable_sort(pl->sorted.begin(), pl->sorted.end(), [](const lmx_batch_problem &x, const lmx_batch_problem &y) {
-            return problem_work(x) > problem_work(y);
-        });
-        ctx();
-        if (count > 0) {
-            LMX_HIP(hipMalloc(&pl->d_probs, sizeof(lmx_batch_problem) * (size_t)count));
-            LMX_HIP(hipMemcpy(pl->d_probs, pl->sorted.data(), sizeof(lmx_batch_problem) * (size_t)count,
-                              hipMemcpyHostToDevice));
-        }
-        LMX_HIP(hipMalloc(&pl->d_counter, 256));
-        pl->args.probs = pl->d_probs;
-        pl->args.count = count;
-        pl->args.counter = pl->d_counter;
-        if (count > 0) {
-            configure(pl.get(), 0);
-            if (flags & LMX_BATCH_AUTOTUNE) {
-                hipStream_t s = ctx().stream;
-                autotune(pl.get(), s);
-                LMX_HIP(hipStreamSynchronize(s));
-            }
+        batch_replan(pl.get(), problems, count, a_pool_elems, b_pool_elems, ipiv_pool_elems, nullptr);
+        if (count > 0 && (flags & LMX_BATCH_AUTOTUNE)) {
+            hipStream_t s = ctx().stream;
+            autotune(pl.get(), s);
+            LMX_HIP(hipStreamSynchronize(s));
         }
         *plan = pl.release();
         return LMX_OK;

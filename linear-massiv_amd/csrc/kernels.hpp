@@ -7,6 +7,8 @@
 
 #include <cstdint>
 
+#include "lmx.h"
+
 namespace lmx {
 
 // C = alpha op(A) op(B) + beta C on MFMA (FP64: v_mfma_f64_16x16x4_f64,
@@ -61,5 +63,44 @@ void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B
 
 // Set *d_info = 0.
 void clear_info(hipStream_t s, int32_t *d_info);
+
+// ---- mixed precision: an FP32 factor of an FP64 matrix, and FP64
+// refinement with it (refine.hip).
+struct RefineOut {
+    int iterations = 0;
+    bool converged = false;
+    bool fell_back = false;
+    int info = 0;
+    double eta = 0.0;
+};
+struct LowFactor {
+    bool spd = false;
+    int64_t n = 0;
+    const double *A = nullptr;  // the FP64 matrix (residuals); must outlive the factor's use
+    int64_t lda = 0;
+    float *A32 = nullptr;       // n x n FP32 factor
+    double *anorm = nullptr;    // ||A||_inf (device)
+    int32_t *ipiv = nullptr, *perm = nullptr, *dinfo = nullptr;
+    int info = 0;
+};
+// Device bytes of a factor of order n (low_factor_bind carves them).
+size_t low_factor_bytes(int64_t n);
+void low_factor_bind(LowFactor &f, void *mem);
+// Factors f.A into f's buffers (synchronises the stream for info).
+void low_factor(hipStream_t s, LowFactor &f);
+// Refines X for B (n x k) with the factor; falls back to an FP64
+// factorisation of f.A when refinement does not converge.
+RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double *B, int64_t ldb, double *X,
+                     int64_t ldx);
+RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_t k, const double *A,
+                       int64_t lda, const double *B, int64_t ldb, double *X, int64_t ldx);
+
+// ---- batched solves (batch.hip): re-targets a plan at a new list of
+// problems, reusing its device buffers (problem table copied on stream s,
+// or synchronously when s is null).
+void batch_replan(lmx_batch_plan *pl, const lmx_batch_problem *problems, int32_t count, int64_t a_pool_elems,
+                  int64_t b_pool_elems, int64_t ipiv_pool_elems, hipStream_t s);
+// Workspace bytes one persistent workgroup needs for problems up to nmax.
+size_t batch_slot_bytes(int nmax, int precision, int flags);
 
 }  // namespace lmx

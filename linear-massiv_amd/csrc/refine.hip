@@ -178,50 +178,61 @@ void launch1d(hipStream_t s, int64_t elems, F launch) {
     if (elems > 0) launch(dim3((unsigned)cdiv(elems, 256)));
 }
 
-struct RefineOut {
-    int iterations = 0;
-    bool converged = false;
-    bool fell_back = false;
-    int info = 0;
-    double eta = 0.0;
-};
+}  // namespace
 
-// precision: LMX_PREC_F32 (factor in FP32).
-RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_t k, const double *A,
-                       int64_t lda, const double *B, int64_t ldb, double *X, int64_t ldx) {
+size_t low_factor_bytes(int64_t n) {
+    return (((size_t)n * (size_t)n * sizeof(float) + 255) / 256) * 256 + (size_t)(2 * n + 8) * sizeof(int32_t) + 256;
+}
+
+void low_factor_bind(LowFactor &f, void *mem) {
+    char *p = static_cast<char *>(mem);
+    const size_t a32 = (((size_t)f.n * (size_t)f.n * sizeof(float) + 255) / 256) * 256;
+    f.A32 = reinterpret_cast<float *>(p);
+    f.anorm = reinterpret_cast<double *>(p + a32);
+    f.ipiv = reinterpret_cast<int32_t *>(p + a32 + 256);
+    f.perm = f.ipiv + f.n;
+    f.dinfo = f.perm + f.n;
+}
+
+void low_factor(hipStream_t s, LowFactor &f) {
+    const int64_t n = f.n;
+    launch1d(s, n * n, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, f.A, f.lda, f.A32, n, n, n); });
+    LMX_LAUNCH_CHECK();
+    if (f.spd)
+        potrf<float>(s, n, f.A32, n, f.dinfo);
+    else
+        getrf_impl<float>(s, n, f.A32, n, f.ipiv, f.perm, f.dinfo);
+    hipLaunchKernelGGL(zero_d_kernel, dim3(1), dim3(256), 0, s, f.anorm, 1);
+    hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, f.A, f.lda, n, f.anorm);
+    LMX_LAUNCH_CHECK();
+    int32_t hinfo = 0;
+    LMX_HIP(hipMemcpyAsync(&hinfo, f.dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    LMX_HIP(hipStreamSynchronize(s));
+    f.info = hinfo;
+}
+
+RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double *B, int64_t ldb, double *X,
+                     int64_t ldx) {
     RefineOut out;
-    if (precision != LMX_PREC_F32) fail_arg("mixed solve: factor precision must be LMX_PREC_F32");
+    const int64_t n = f.n;
+    out.info = f.info;
     if (n == 0 || k == 0) {
         out.converged = true;
         return out;
     }
-    TimedRegion trall("refine", s);
-    const size_t nn = (size_t)n * (size_t)n, nk = (size_t)n * (size_t)k;
-    Scratch sA32(nn * sizeof(float), s), sR(nk * sizeof(double), s), sR32(nk * sizeof(float), s),
-        sI((size_t)(2 * n + 8) * sizeof(int32_t), s), sN((size_t)(3 * k + 8) * sizeof(double), s);
-    float *A32 = sA32.as<float>(), *R32 = sR32.as<float>();
+    const size_t nk = (size_t)n * (size_t)k;
+    Scratch sR(nk * sizeof(double), s), sR32(nk * sizeof(float), s), sN((size_t)(3 * k + 8) * sizeof(double), s),
+        sI((size_t)(2 * n + 8) * sizeof(int32_t), s);
+    float *R32 = sR32.as<float>();
     double *R = sR.as<double>(), *norms = sN.as<double>();
-    int32_t *ipiv = sI.as<int32_t>(), *perm = ipiv + n, *dinfo = perm + n;
+    const double *A = f.A;
+    const int64_t lda = f.lda;
     const int K = (int)k;
-
-    launch1d(s, (int64_t)nn, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, A, lda, A32, n, n, n); });
-    LMX_LAUNCH_CHECK();
-    if (spd)
-        potrf<float>(s, n, A32, n, dinfo);
-    else
-        getrf_impl<float>(s, n, A32, n, ipiv, perm, dinfo);
-    int32_t hinfo = 0;
-    LMX_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    LMX_HIP(hipStreamSynchronize(s));
-    out.info = hinfo;
-
     const double crit = std::sqrt((double)n) * EPS64;
-    bool lowprec_ok = (hinfo == 0);
-    if (lowprec_ok) {
+    if (f.info == 0) {
         hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
         LMX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, A, lda, n, norms);
-        LMX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(copy_d_kernel, dim3(1), dim3(256), 0, s, f.anorm, 1, norms, 1, 1, 1);
         colmax(s, B, ldb, n, K, norms + 1);
         LMX_LAUNCH_CHECK();
         // R = B
@@ -229,10 +240,10 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
         double prev = INFINITY;
         for (int it = 0; it < ITMAX; ++it) {
             launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, R, k, R32, k, n, k); });
-            if (spd)
-                potrs<float>(s, n, k, A32, n, R32, k);
+            if (f.spd)
+                potrs<float>(s, n, k, f.A32, n, R32, k);
             else
-                getrs_perm<float>(s, n, k, A32, n, perm, R32, k);
+                getrs_perm<float>(s, n, k, f.A32, n, f.perm, R32, k);
             launch1d(s, (int64_t)nk, [&](dim3 g) {
                 hipLaunchKernelGGL(f2d_acc_kernel, g, dim3(256), 0, s, R32, k, X, ldx, n, k, it == 0 ? 1 : 0);
             });
@@ -259,11 +270,13 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
     if (!out.converged) {
         // FP64 factorisation of a copy of A.
         out.fell_back = true;
+        const size_t nn = (size_t)n * (size_t)n;
         Scratch sA64(nn * sizeof(double), s);
         double *A64 = sA64.as<double>();
+        int32_t *ipiv = sI.as<int32_t>(), *perm = ipiv + n, *dinfo = perm + n;
         launch1d(s, (int64_t)nn, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, A, lda, A64, n, n, n); });
         launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, X, ldx, n, k); });
-        if (spd) {
+        if (f.spd) {
             potrf<double>(s, n, A64, n, dinfo);
             potrs<double>(s, n, k, A64, n, X, ldx);
         } else {
@@ -290,6 +303,29 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
     }
     return out;
 }
+
+// precision: LMX_PREC_F32 (factor in FP32).
+RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_t k, const double *A,
+                       int64_t lda, const double *B, int64_t ldb, double *X, int64_t ldx) {
+    RefineOut out;
+    if (precision != LMX_PREC_F32) fail_arg("mixed solve: factor precision must be LMX_PREC_F32");
+    if (n == 0 || k == 0) {
+        out.converged = true;
+        return out;
+    }
+    TimedRegion trall("refine", s);
+    Scratch mem(low_factor_bytes(n), s);
+    LowFactor f;
+    f.spd = spd;
+    f.n = n;
+    f.A = A;
+    f.lda = lda;
+    low_factor_bind(f, mem.p);
+    low_factor(s, f);
+    return low_refine(s, f, k, B, ldb, X, ldx);
+}
+
+namespace {
 
 void fill(lmx_refine_info *info, const RefineOut &o) {
     if (!info) return;

@@ -61,11 +61,14 @@ EXPORTED = [
     "lmx_dgesv_mixed", "lmx_dgesv_mixed_dev", "lmx_dposv_mixed", "lmx_dposv_mixed_dev",
     "lmx_batch_plan_create", "lmx_batch_plan_execute", "lmx_batch_plan_autotune", "lmx_batch_plan_query",
     "lmx_batch_plan_destroy",
+    "lmx_server_create", "lmx_server_submit", "lmx_server_wait", "lmx_server_poll", "lmx_server_factor_release",
+    "lmx_server_stats", "lmx_server_destroy",
     "lmx_kernel_timing", "lmx_kernel_timing_reset", "lmx_kernel_timing_get",
 ]
 BATCH_MAX_N, BATCH_MAX_NRHS = 512, 8
 BATCH_LU, BATCH_CHOLESKY = 0, 1
 BATCH_FACTORS, BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2, 4
+REQ_KEEP_FACTOR = 1
 
 
 class LmxError(RuntimeError):
@@ -85,6 +88,26 @@ class BatchProblem(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("n", ctypes.c_int32), ("nrhs", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("a_offset", ctypes.c_int64), ("b_offset", ctypes.c_int64),
                 ("ipiv_offset", ctypes.c_int64)]
+
+
+class ServerConfig(ctypes.Structure):
+    """lmx_server_config (include/lmx.h)."""
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int32), ("small_n", ctypes.c_int32),
+                ("small_max", ctypes.c_int32), ("reserved", ctypes.c_int32), ("aging_ms", ctypes.c_double)]
+
+
+class RequestResult(ctypes.Structure):
+    """lmx_request_result (include/lmx.h)."""
+    _fields_ = [("status", ctypes.c_int32), ("rc", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("fell_back", ctypes.c_int32), ("backward_error", ctypes.c_double), ("factor", ctypes.c_int64),
+                ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class ServerCounters(ctypes.Structure):
+    """lmx_server_counters (include/lmx.h)."""
+    _fields_ = [(f, ctypes.c_int64) for f in ("completed", "failed", "queued", "inflight", "bytes_in_use",
+                                              "peak_bytes", "budget_bytes", "resident_factors")] + \
+               [("busy_ms", ctypes.c_double), ("window_ms", ctypes.c_double)]
 
 
 _i64, _dbl, _flt, _vp, _chr = ctypes.c_int64, ctypes.c_double, ctypes.c_float, ctypes.c_void_p, ctypes.c_char
@@ -122,6 +145,13 @@ _sigs = {
     "lmx_batch_plan_query": [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
                              ctypes.POINTER(_i32), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)],
     "lmx_batch_plan_destroy": [_vp],
+    "lmx_server_create": [ctypes.POINTER(_vp), ctypes.POINTER(ServerConfig)],
+    "lmx_server_submit": [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i32, ctypes.POINTER(_i64)],
+    "lmx_server_wait": [_vp, _i64, ctypes.POINTER(RequestResult)],
+    "lmx_server_poll": [_vp, _i64, ctypes.POINTER(RequestResult)],
+    "lmx_server_factor_release": [_vp, _i64],
+    "lmx_server_stats": [_vp, ctypes.POINTER(ServerCounters), _i32],
+    "lmx_server_destroy": [_vp],
     "lmx_kernel_timing": [ctypes.c_int], "lmx_kernel_timing_reset": [],
     "lmx_kernel_timing_get": [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)],
 }
@@ -528,6 +558,69 @@ def solve_batched(problems, precision=PREC_F64, flags=0, return_factors=False):
                for p in probs]
         out = out + (fac,)
     return out
+
+
+# ---------------------------------------------------------------- solve server
+
+class Server:
+    """The request loop of lmx_server_* on host numpy buffers.  submit()
+    keeps the buffers alive until the request's result is collected."""
+
+    def __init__(self, budget_bytes=0, slots=4, small_n=0, small_max=0, aging_ms=0.0):
+        cfg = ServerConfig(budget_bytes, slots, small_n, small_max, 0, aging_ms)
+        h = _vp()
+        _check(_lib.lmx_server_create(ctypes.byref(h), ctypes.byref(cfg)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, kind, B, X, A=None, factor=0, keep=False):
+        """A x = B for host arrays (row-major float64; X is written).  Either
+        A or a resident factor handle of the same kind and order."""
+        n, nrhs = B.shape
+        for arr in (B, X) + ((A,) if A is not None else ()):
+            if arr.dtype != np.float64 or not arr.flags["C_CONTIGUOUS"]:
+                raise ValueError("server buffers must be C-contiguous float64")
+        rid = _i64()
+        _check(_lib.lmx_server_submit(self._h, kind, n, nrhs, _ptr(A) if A is not None else None,
+                                      A.shape[1] if A is not None else 0, factor, _ptr(B), nrhs, _ptr(X), nrhs,
+                                      REQ_KEEP_FACTOR if keep else 0, ctypes.byref(rid)))
+        self._keep[rid.value] = (A, B, X)
+        return rid.value
+
+    def wait(self, rid):
+        res = RequestResult()
+        _check(_lib.lmx_server_wait(self._h, rid, ctypes.byref(res)))
+        self._keep.pop(rid, None)
+        return res
+
+    def poll(self, rid):
+        res = RequestResult()
+        rc = _lib.lmx_server_poll(self._h, rid, ctypes.byref(res))
+        if rc == 1:
+            return None
+        _check(rc)
+        self._keep.pop(rid, None)
+        return res
+
+    def release(self, factor):
+        _check(_lib.lmx_server_factor_release(self._h, factor))
+
+    def stats(self, reset=False):
+        st = ServerCounters()
+        _check(_lib.lmx_server_stats(self._h, ctypes.byref(st), 1 if reset else 0))
+        return {f: getattr(st, f) for f, _ in ServerCounters._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(_lib.lmx_server_destroy(self._h))
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- kernel timing

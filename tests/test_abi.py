@@ -100,3 +100,21 @@ def test_batch_layout_packs_back_to_back():
     assert [p.ipiv_offset for p in probs] == [0, 3, 8]
     assert (na, nb, npv) == (38, 27, 10)
     assert ctypes.sizeof(lmx.BatchProblem) == 40
+
+
+@pytest.mark.parametrize("kw,msg", [({"slots": 17}, "slots"), ({"small_n": 600}, "small_n"),
+                                    ({"budget_bytes": -1}, "budget")])
+def test_server_config_validation(kw, msg):
+    # checked before any device call
+    import lmx
+    with pytest.raises(lmx.LmxError) as e:
+        lmx.Server(**kw)
+    assert e.value.code == lmx.LMX_ERR_ARG
+    assert msg in str(e.value)
+
+
+def test_server_structs_match_the_header():
+    import lmx
+    assert ctypes.sizeof(lmx.ServerConfig) == 32
+    assert ctypes.sizeof(lmx.RequestResult) == 56
+    assert ctypes.sizeof(lmx.ServerCounters) == 80
