@@ -46,23 +46,265 @@ constexpr int MAXP = 64;      // max cooperative workgroups (panel height <= 327
 constexpr int NB_OUTER = 128; // outer panel width (k of the trailing GEMM)
 constexpr long SPIN_LIMIT = 1L << 22;
 
-// Cross-workgroup state of the panel kernel.  seq[w] counts the columns
-// workgroup w has published (monotonic over a whole factorisation, so it
-// needs clearing once per getrf); candidates are double-buffered by column
-// parity.
+// Cross-workgroup records of the panel kernel.  Every word is 64 bits: a
+// 32-bit payload and, in the upper half, the tag base + j + 1 of the column
+// it belongs to, written with one device-coherent store; a reader polls a
+// word until its tag matches, so the poll is the load and no flag is needed.
+// Tags grow over the whole factorisation (the records are zeroed once per
+// getrf) and records are double-buffered by column parity.
+//   workgroup record: value (WPV words), row, tag, pad to even, row data
+//   row-j record:     tag, pad, row data
+template <class T>
+struct PanelRec {
+    static constexpr int WPV = sizeof(T) / 4;
+    static constexpr int DATA = WPV + 2 + (WPV == 2 ? 0 : 0);  // data offset (even for doubles)
+    static constexpr int W = DATA + KB * WPV;                  // words per workgroup record
+    static constexpr int DDATA = 2;
+    static constexpr int DW = DDATA + KB * WPV;                // words of the row-j record
+    static constexpr int TOTAL = MAXP * W + DW;
+};
 template <class T>
 struct PanelSync {
-    unsigned seq[MAXP];
+    unsigned long long rec[2][MAXP * PanelRec<T>::W];
+    unsigned long long drow[2][PanelRec<T>::DW];
     unsigned error;
-    unsigned pad[15];
-    T cand_val[2][MAXP];
-    int cand_row[2][MAXP];
-    int cand_tag[2][MAXP];
-    T cand_data[2][MAXP][KB];
-    T diag_data[2][KB];
-    int diag_tag[2];
+    unsigned pad[63];
 };
-constexpr int SYNC_CLEAR_WORDS = MAXP + 16;
+template <class T>
+constexpr int sync_clear_words() {
+    return (int)(sizeof(PanelSync<T>) / 4);
+}
+
+__device__ inline unsigned long long tagword(unsigned payload, unsigned tag) {
+    return (unsigned long long)payload | ((unsigned long long)tag << 32);
+}
+template <class T>
+__device__ inline unsigned word_of(T v, int h) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(unsigned, v);
+    else
+        return h == 0 ? (unsigned)__builtin_bit_cast(unsigned long long, v)
+                      : (unsigned)(__builtin_bit_cast(unsigned long long, v) >> 32);
+}
+
+constexpr int PG = 4;  // columns per register rotation
+
+// Per-thread state of the panel kernel: RPT rows of the panel in registers,
+// in the rotated frame (register c holds column j0 + c of the current
+// group), each with its panel-relative origin (tag).
+template <class T>
+struct PanelRows {
+    T v[RPT][KB];
+    int row[RPT];
+    int tag[RPT];
+};
+
+// LDS of the panel kernel (double-buffered by column parity).
+template <class T>
+struct PanelLds {
+    T slot_data[2][PW][KB];        // each wave's best row
+    T slot_val[2][PW];
+    int slot_row[2][PW];
+    int slot_tag[2][PW];
+    T drow[2][KB];                 // row j (single workgroup)
+    int dtag[2];
+    int pivs[KB];
+    alignas(8) unsigned pay[2][PanelRec<T>::TOTAL];  // polled record payloads (several workgroups)
+};
+
+// One column (j = j0 + U) of the panel: pivot search, interchange, scaling
+// and rank-1 update of the rest of the panel.  Returns the pivot row.
+template <class T, int U>
+__device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, int j0, int kb, int m, int P,
+                                             PanelSync<T> *__restrict__ sy, unsigned base, int k0,
+                                             int32_t *__restrict__ ipiv, int &local_info, bool &failed) {
+    typedef PanelRec<T> R;
+    const int j = j0 + U;
+    if (j >= kb) return;
+    const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int par = j & 1;
+    // 1. this wave's candidate: max |a_ij| over its rows i >= j, then the
+    //    lowest row attaining it; its owner writes the row to the wave slot
+    T a[RPT];
+    T tmax = T(-1);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        a[q] = (st.row[q] >= j && st.row[q] < m) ? fabs(st.v[q][U]) : T(-1);
+        tmax = fmax(tmax, a[q]);
+    }
+    const T wmax = wave_max(tmax);
+    int trow = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+        if (a[q] == wmax && a[q] >= T(0)) trow = min(trow, st.row[q]);
+    const int wrow = wave_min(trow);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        if (st.row[q] == wrow) {
+#pragma unroll
+            for (int c = 0; c < KB; ++c) L.slot_data[par][wave][c] = st.v[q][c];
+            L.slot_tag[par][wave] = st.tag[q];
+        }
+        if (st.row[q] == j) {
+#pragma unroll
+            for (int c = 0; c < KB; ++c) L.drow[par][c] = st.v[q][c];
+            L.dtag[par] = st.tag[q];
+        }
+    }
+    if (lane == 0) {
+        L.slot_val[par][wave] = wmax;
+        L.slot_row[par][wave] = wrow;
+    }
+    __syncthreads();
+    // 2. this workgroup's candidate
+    T best = L.slot_val[par][0];
+#pragma unroll
+    for (int x = 1; x < PW; ++x) best = fmax(best, L.slot_val[par][x]);
+    int bw = -1, brow = INT_MAX;
+#pragma unroll
+    for (int x = 0; x < PW; ++x)
+        if (L.slot_val[par][x] == best && L.slot_row[par][x] < brow) {
+            brow = L.slot_row[par][x];
+            bw = x;
+        }
+    if (!(best >= T(0))) {
+        bw = -1;
+        brow = INT_MAX;
+    }
+    int p, ptag, dtag;
+    const T *prow;
+    const T *drow;
+    if (P == 1) {
+        drow = L.drow[par];
+        dtag = L.dtag[par];
+        // a NaN at row j keeps row j, as rawPivotSearch does
+        if (bw < 0 || isnan(drow[U])) {
+            p = j;
+            prow = drow;
+            ptag = dtag;
+        } else {
+            p = brow;
+            prow = L.slot_data[par][bw];
+            ptag = L.slot_tag[par][bw];
+        }
+    } else {
+        const unsigned tg = base + (unsigned)j + 1u;
+        // 3. publish this workgroup's record (and row j from workgroup 0)
+        if (wave == 0) {
+            unsigned long long *rec = sy->rec[par] + w * R::W;
+            for (int e = lane; e < R::W; e += 64) {
+                unsigned pl = 0;
+                if (e < R::WPV)
+                    pl = word_of(best, e);
+                else if (e == R::WPV)
+                    pl = (unsigned)brow;
+                else if (e == R::WPV + 1)
+                    pl = bw >= 0 ? (unsigned)L.slot_tag[par][bw] : 0u;
+                else if (e >= R::DATA)
+                    pl = bw >= 0 ? word_of(L.slot_data[par][bw][(e - R::DATA) / R::WPV], (e - R::DATA) % R::WPV) : 0u;
+                xstore(rec + e, tagword(pl, tg));
+            }
+            if (w == 0)
+                for (int e = lane; e < R::DW; e += 64) {
+                    unsigned pl = 0;
+                    if (e == 0)
+                        pl = (unsigned)L.dtag[par];
+                    else if (e >= R::DDATA)
+                        pl = word_of(L.drow[par][(e - R::DDATA) / R::WPV], (e - R::DDATA) % R::WPV);
+                    xstore(sy->drow[par] + e, tagword(pl, tg));
+                }
+        }
+        // 4. poll every record: all loads in flight first, then the
+        //    stragglers one by one
+        constexpr int NPOLL = (R::TOTAL + PT - 1) / PT;
+        const int total = P * R::W + R::DW;
+        unsigned long long wv[NPOLL];
+#pragma unroll
+        for (int i = 0; i < NPOLL; ++i) {
+            const int e = t + i * PT;
+            if (e < total) {
+                const unsigned long long *wp = e < P * R::W ? sy->rec[par] + e : sy->drow[par] + (e - P * R::W);
+                wv[i] = xload(wp);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NPOLL; ++i) {
+            const int e = t + i * PT;
+            if (e < total) {
+                const unsigned long long *wp = e < P * R::W ? sy->rec[par] + e : sy->drow[par] + (e - P * R::W);
+                long spins = 0;
+                while ((unsigned)(wv[i] >> 32) != tg && !failed) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 255) == 0 && xload(&sy->error) != 0u) failed = true;
+                    if (spins > SPIN_LIMIT) {
+                        xstore(&sy->error, 1u);
+                        failed = true;
+                    }
+                    wv[i] = xload(wp);
+                }
+                const int dst = e < P * R::W ? e : MAXP * R::W + (e - P * R::W);
+                L.pay[par][dst] = (unsigned)wv[i];
+            }
+        }
+        __syncthreads();
+        // 5. the pivot: largest value over the workgroups, lowest row on
+        //    ties; every wave reduces the P records itself (lane x reads
+        //    record x), so no serial loop and no barrier
+        const unsigned *pay = L.pay[par];
+        const T xv = lane < P ? *reinterpret_cast<const T *>(pay + lane * R::W) : T(-1);
+        const int xr = lane < P ? (int)pay[lane * R::W + R::WPV] : INT_MAX;
+        const T gb = wave_max(xv);
+        const int gr = wave_min((gb >= T(0) && xv == gb) ? xr : INT_MAX);
+        // the workgroup that owns the pivot row (rows are split in
+        // PANEL_ROWS-row blocks)
+        const int gw = gr == INT_MAX ? -1 : gr / PANEL_ROWS;
+        drow = reinterpret_cast<const T *>(pay + MAXP * R::W + R::DDATA);
+        dtag = (int)pay[MAXP * R::W];
+        if (gw < 0 || isnan(drow[U])) {
+            p = j;
+            prow = drow;
+            ptag = dtag;
+        } else {
+            p = gr;
+            prow = reinterpret_cast<const T *>(pay + gw * R::W + R::DATA);
+            ptag = (int)pay[gw * R::W + R::WPV + 1];
+        }
+    }
+    // 6. interchange rows j and p
+    if (p != j) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (st.row[q] == p) {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) st.v[q][c] = drow[c];
+                st.tag[q] = dtag;
+            } else if (st.row[q] == j) {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) st.v[q][c] = prow[c];
+                st.tag[q] = ptag;
+            }
+        }
+    }
+    if (t == 0) L.pivs[j] = p;
+    if (w == 0 && t == 0) ipiv[k0 + j] = k0 + p;
+    // 7. multipliers (a division, as the reference) and the rank-1 update
+    //    of the panel columns after j
+    const T piv = prow[U];
+    const int lim = kb - j0;
+    if (piv != T(0)) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (st.row[q] > j && st.row[q] < m) {
+                const T l = st.v[q][U] / piv;
+                st.v[q][U] = l;
+#pragma unroll
+                for (int c = U + 1; c < KB; ++c)
+                    if (c < lim) st.v[q][c] = fma(-l, prow[c], st.v[q][c]);
+            }
+    } else if (local_info == 0) {
+        local_info = k0 + j + 1;
+    }
+}
 
 template <class T>
 __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
@@ -72,213 +314,60 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
                                                           int32_t *__restrict__ info,
                                                           PanelSync<T> *__restrict__ sy, int P,
                                                           unsigned base) {
-    const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    __shared__ T s_cand[KB], s_drow[KB], s_prow[KB];
-    __shared__ int s_ctag, s_dtag;
-    __shared__ T s_wbest[PW];
-    __shared__ int s_wrow[PW];
-    __shared__ int s_pivs[KB];
-    __shared__ T s_val[MAXP];
-    __shared__ int s_row[MAXP], s_tag[MAXP];
-    __shared__ T s_all[MAXP][KB + 1];
-
-    T v[RPT][KB];
-    int tag[RPT], row[RPT];
+    const int w = blockIdx.x, t = threadIdx.x;
+    __shared__ PanelLds<T> L;
+    PanelRows<T> st;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        row[q] = w * PANEL_ROWS + q * PT + t;
-        tag[q] = row[q];
+        st.row[q] = w * PANEL_ROWS + q * PT + t;
+        st.tag[q] = st.row[q];
 #pragma unroll
         for (int c = 0; c < KB; ++c)
-            v[q][c] = (row[q] < m && c < kb) ? A[(int64_t)row[q] * lda + c] : T(0);
+            st.v[q][c] = (st.row[q] < m && c < kb) ? A[(int64_t)st.row[q] * lda + c] : T(0);
     }
     int local_info = 0;
     bool failed = false;
-
-    for (int j = 0; j < kb; ++j) {
-        const int par = j & 1;
-        // 1. candidate of this workgroup: max |a_ij| over its rows i >= j,
-        //    then the lowest row attaining it (wave DPP reductions, then
-        //    one LDS exchange between the waves)
-        T a[RPT];
-        T tmax = T(-1);
+    int groups = 0;
+    for (int j0 = 0; j0 < kb; j0 += PG) {
+        panel_column<T, 0>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 1>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 2>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 3>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        // rotate: the next group's first column moves to register 0
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            a[q] = (row[q] >= j && row[q] < m) ? fabs(v[q][0]) : T(-1);
-            tmax = fmax(tmax, a[q]);
+            T keep[PG];
+#pragma unroll
+            for (int c = 0; c < PG; ++c) keep[c] = st.v[q][c];
+#pragma unroll
+            for (int c = 0; c < KB - PG; ++c) st.v[q][c] = st.v[q][c + PG];
+#pragma unroll
+            for (int c = 0; c < PG; ++c) st.v[q][KB - PG + c] = keep[c];
         }
-        const T wmax = wave_max(tmax);
-        int trow = INT_MAX;
-#pragma unroll
-        for (int q = 0; q < RPT; ++q)
-            if (a[q] == wmax && row[q] >= j && row[q] < m) trow = min(trow, row[q]);
-        const int wrow = wave_min(trow);
-        if (lane == 0) {
-            s_wbest[wave] = wmax;
-            s_wrow[wave] = wrow;
-        }
-        __syncthreads();
-        T best = s_wbest[0];
-#pragma unroll
-        for (int x = 1; x < PW; ++x) best = fmax(best, s_wbest[x]);
-        int brow = INT_MAX;
-#pragma unroll
-        for (int x = 0; x < PW; ++x)
-            if (s_wbest[x] == best) brow = min(brow, s_wrow[x]);
-        if (!(best >= T(0))) brow = INT_MAX;  // no row of this workgroup is a candidate
-        // 2. stage the candidate row and row j (always in workgroup 0) in LDS
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            if (row[q] == brow) {
-#pragma unroll
-                for (int c = 0; c < KB; ++c) s_cand[c] = v[q][c];
-                s_ctag = tag[q];
-            }
-            if (row[q] == j) {
-#pragma unroll
-                for (int c = 0; c < KB; ++c) s_drow[c] = v[q][c];
-                s_dtag = tag[q];
-            }
-        }
-        __syncthreads();
-
-        int p, ptag;
-        if (P > 1) {
-            // 3. publish: device-coherent stores by wave 0, drained, then the
-            //    sequence number
-            if (wave == 0) {
-                if (lane < KB) xstore(&sy->cand_data[par][w][lane], s_cand[lane]);
-                if (lane == KB) xstore(&sy->cand_val[par][w], best);
-                if (lane == KB + 1) xstore(&sy->cand_row[par][w], brow);
-                if (lane == KB + 2) xstore(&sy->cand_tag[par][w], s_ctag);
-                if (w == 0) {
-                    if (lane < KB) xstore(&sy->diag_data[par][lane], s_drow[lane]);
-                    if (lane == KB + 3) xstore(&sy->diag_tag[par], s_dtag);
-                }
-                xdrain();
-                if (lane == 0) xstore(&sy->seq[w], base + (unsigned)j + 1u);
-            }
-            // 4. wait until every workgroup has published column j
-            if (t < P && !failed) {
-                const unsigned want = base + (unsigned)j + 1u;
-                long spins = 0;
-                while (xload(&sy->seq[t]) < want) {
-                    if (xload(&sy->error) != 0u) {
-                        failed = true;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > SPIN_LIMIT) {
-                        xstore(&sy->error, 1u);
-                        failed = true;
-                        break;
-                    }
-                }
-            }
-            __syncthreads();
-            // 5. fetch every candidate record with device-coherent loads
-            for (int e = t; e < P; e += PT) {
-                s_val[e] = xload(&sy->cand_val[par][e]);
-                s_row[e] = xload(&sy->cand_row[par][e]);
-                s_tag[e] = xload(&sy->cand_tag[par][e]);
-            }
-#pragma unroll
-            for (int i = 0; i < MAXP * KB / PT; ++i) {
-                const int e = t + i * PT;
-                if (e < P * KB) s_all[e / KB][e % KB] = xload(&sy->cand_data[par][e / KB][e % KB]);
-            }
-            if (w != 0) {
-                if (t < KB) s_drow[t] = xload(&sy->diag_data[par][t]);
-                if (t == 0) s_dtag = xload(&sy->diag_tag[par]);
-            }
-            __syncthreads();
-            // 6. the global pivot, from LDS (same rule as within a workgroup)
-            T b = T(-1);
-            for (int x = 0; x < P; ++x) b = fmax(b, s_val[x]);
-            int r = INT_MAX, ws = -1;
-            for (int x = 0; x < P; ++x)
-                if (b >= T(0) && s_val[x] == b && s_row[x] < r) {
-                    r = s_row[x];
-                    ws = x;
-                }
-            // a NaN at row j keeps row j, as rawPivotSearch does
-            if (isnan(s_drow[0])) ws = -1;
-            p = (ws < 0) ? j : r;
-            if (t < KB) s_prow[t] = (ws >= 0) ? s_all[ws][t] : s_drow[t];
-            ptag = (ws >= 0) ? s_tag[ws] : s_dtag;
-        } else {
-            const bool own = brow != INT_MAX && !isnan(s_drow[0]);
-            p = own ? brow : j;
-            if (t < KB) s_prow[t] = own ? s_cand[t] : s_drow[t];
-            ptag = own ? s_ctag : s_dtag;
-        }
-        __syncthreads();
-        const T *prow = s_prow;
-
-        // 7. interchange rows j and p
-        if (p != j) {
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                if (row[q] == p) {
-#pragma unroll
-                    for (int c = 0; c < KB; ++c) v[q][c] = s_drow[c];
-                    tag[q] = s_dtag;
-                } else if (row[q] == j) {
-#pragma unroll
-                    for (int c = 0; c < KB; ++c) v[q][c] = s_prow[c];
-                    tag[q] = ptag;
-                }
-            }
-        }
-        if (t == 0) s_pivs[j] = p;
-        if (w == 0 && t == 0) ipiv[k0 + j] = k0 + p;
-
-        // 8. multipliers and rank-1 update of the remaining panel columns
-        const T piv = prow[0];
-        if (piv != T(0)) {
-            const int lim = kb - j;
-#pragma unroll
-            for (int q = 0; q < RPT; ++q)
-                if (row[q] > j && row[q] < m) {
-                    T l = v[q][0] / piv;
-                    v[q][0] = l;
-#pragma unroll
-                    for (int c = 1; c < KB; ++c)
-                        if (c < lim) v[q][c] = fma(-l, prow[c], v[q][c]);
-                }
-        } else if (local_info == 0) {
-            local_info = k0 + j + 1;
-        }
-        // 9. rotate: the next column moves to register 0
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            T f = v[q][0];
-#pragma unroll
-            for (int c = 0; c < KB - 1; ++c) v[q][c] = v[q][c + 1];
-            v[q][KB - 1] = f;
-        }
-        __syncthreads();
+        ++groups;
     }
+    __syncthreads();
 
-    // After kb rotations register s holds column (s + kb) % KB.
+    // After the rotations register s holds column (s + PG*groups) % KB.
+    const int shift = (PG * groups) % KB;
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
-        if (row[q] < m) {
+        if (st.row[q] < m) {
 #pragma unroll
-            for (int s = 0; s < KB; ++s) {
-                int c = (s + kb) % KB;
-                if (c < kb) A[(int64_t)row[q] * lda + c] = v[q][s];
+            for (int s2 = 0; s2 < KB; ++s2) {
+                int c = (s2 + shift) % KB;
+                if (c < kb) A[(int64_t)st.row[q] * lda + c] = st.v[q][s2];
             }
-            srcmap[row[q]] = tag[q];
+            srcmap[st.row[q]] = st.tag[q];
         }
     if (w == 0 && t == 0) {
         int cnt = 0;
         for (int d = 0; d < kb && d < m; ++d) moves[1 + cnt++] = d;
         for (int j = 0; j < kb; ++j) {
-            int pj = s_pivs[j];
+            int pj = L.pivs[j];
             if (pj < kb) continue;
             bool dup = false;
-            for (int x = 0; x < j; ++x) dup |= (s_pivs[x] == pj);
+            for (int x = 0; x < j; ++x) dup |= (L.pivs[x] == pj);
             if (!dup) moves[1 + cnt++] = pj;
         }
         moves[0] = cnt;
@@ -438,7 +527,8 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
     auto *sy = scr.as<PanelSync<T>>();
     int32_t *srcmap = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scr.p) + sync_bytes);
     int32_t *moves = srcmap + N;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, s, (unsigned *)sy, SYNC_CLEAR_WORDS);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)cdiv(sync_clear_words<T>(), 256)), dim3(256), 0, s,
+                       (unsigned *)sy, sync_clear_words<T>());
     LMX_LAUNCH_CHECK();
     if (d_perm) {
         hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, s, d_perm, N);

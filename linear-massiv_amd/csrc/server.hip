@@ -109,7 +109,8 @@ int64_t req_need(const Request &r, const Factor *reuse) {
     const int64_t n = r.n, k = r.nrhs;
     int64_t b = n * k * (8 + 8 + 8 + 4) + n * 64 * 4 + (4 << 20);  // B, X, R, R32, TRSM blocks, scratch
     if (r.small) return n * n * 12 + b;                                // matrix + FP32 copy, batched
-    if (!reuse) b += n * n * 8 + (int64_t)low_factor_bytes(n);        // matrix + FP32 factor
+    if (!reuse) b += n * n * 8;                                        // the matrix
+    if (!reuse || !reuse->factored) b += (int64_t)low_factor_bytes(n); // an FP32 factor to compute
     b += n * n * 8;                                                    // FP64 fallback copy
     return b;
 }
@@ -171,8 +172,26 @@ void run_one(lmx_server *sv, Request *r, Factor *reuse, hipStream_t s) {
     try {
         copy_in(dB, k, r->B, r->ldb, n, k, s);
         RefineOut o;
-        if (reuse) {
+        if (reuse && reuse->factored) {
             o = low_refine(s, reuse->f, k, dB, k, dX, k);
+        } else if (reuse) {
+            // a resident matrix kept without a factor (it came through the
+            // batched path): factor it for this request only
+            LowFactor tf;
+            tf.spd = reuse->kind == LMX_BATCH_CHOLESKY;
+            tf.n = n;
+            tf.A = reuse->A64;
+            tf.lda = n;
+            void *mem = dalloc<char>(low_factor_bytes(n), s);
+            low_factor_bind(tf, mem);
+            try {
+                low_factor(s, tf);
+                o = low_refine(s, tf, k, dB, k, dX, k);
+            } catch (...) {
+                hip_free_async(mem, s);
+                throw;
+            }
+            hip_free_async(mem, s);
         } else {
             std::unique_ptr<Factor> f(new Factor());
             f->kind = r->kind;
@@ -197,9 +216,10 @@ void run_one(lmx_server *sv, Request *r, Factor *reuse, hipStream_t s) {
             if (r->keep) {
                 f->bytes = factor_bytes(n, true);
                 std::lock_guard<std::mutex> g(sv->mu);
-                f->id = sv->next_id++;
-                r->factor_out = f->id;
-                sv->factors[f->id] = f.release();
+                Factor *fp = f.release();
+                fp->id = sv->next_id++;
+                r->factor_out = fp->id;
+                sv->factors[fp->id] = fp;
             } else {
                 hip_free_async(f->A64, s);
                 hip_free_async(f->mem, s);
@@ -234,7 +254,9 @@ void run_small(lmx_server *sv, std::vector<Request *> &grp, std::vector<Factor *
     double *dA = dalloc<double>((size_t)ao, s);
     double *dB = dalloc<double>((size_t)bo, s);
     int32_t *dI = dalloc<int32_t>(2 * grp.size(), s);
+    std::vector<int32_t> hI(2 * grp.size());
     auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);  // no copy into hI may outlive it
         hip_free_async(dA, s);
         hip_free_async(dB, s);
         hip_free_async(dI, s);
@@ -253,7 +275,6 @@ void run_small(lmx_server *sv, std::vector<Request *> &grp, std::vector<Factor *
         batch_replan(plan, probs.data(), (int32_t)probs.size(), ao, bo, 0, s);
         const int rc = lmx_batch_plan_execute(plan, dA, dB, nullptr, dI, dI + grp.size(), s);
         if (rc != LMX_OK) throw Error(rc, lmx_last_error());
-        std::vector<int32_t> hI(2 * grp.size());
         LMX_HIP(hipMemcpyAsync(hI.data(), dI, hI.size() * 4, hipMemcpyDeviceToHost, s));
         for (size_t i = 0; i < grp.size(); ++i)
             copy_out(grp[i]->X, grp[i]->ldx, dB + probs[i].b_offset, grp[i]->nrhs, grp[i]->n, grp[i]->nrhs, s);
