@@ -289,7 +289,12 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     if (w == 0 && t == 0) ipiv[k0 + j] = k0 + p;
     // 7. multipliers (a division, as the reference) and the rank-1 update
     //    of the panel columns after j
-    const T piv = prow[U];
+    // the pivot row is read from LDS in one batch first: a read under the
+    // per-column guard below would wait for each load in turn
+    T pr[KB];
+#pragma unroll
+    for (int c = U; c < KB; ++c) pr[c] = prow[c];
+    const T piv = pr[U];
     const int lim = kb - j0;
     if (piv != T(0)) {
 #pragma unroll
@@ -299,7 +304,7 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
                 st.v[q][U] = l;
 #pragma unroll
                 for (int c = U + 1; c < KB; ++c)
-                    if (c < lim) st.v[q][c] = fma(-l, prow[c], st.v[q][c]);
+                    if (c < lim) st.v[q][c] = fma(-l, pr[c], st.v[q][c]);
             }
     } else if (local_info == 0) {
         local_info = k0 + j + 1;
