@@ -100,16 +100,24 @@ void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info) {
 }
 
 template <class T>
-void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb) {
+void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb, const T *WL,
+           const T *WU) {
     if (n == 0 || nrhs == 0) return;
+    if (WL && WU) {
+        tri_inverse_solve<T>(s, true, false, n, nrhs, L, lda, WL, B, ldb);
+        tri_inverse_solve<T>(s, true, true, n, nrhs, L, lda, WU, B, ldb);
+        return;
+    }
     trsm_left<T>(s, true, false, false, n, nrhs, L, lda, B, ldb);
     trsm_left<T>(s, true, true, false, n, nrhs, L, lda, B, ldb);
 }
 
 template void potrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *);
 template void potrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *);
-template void potrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, double *, int64_t);
-template void potrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, float *, int64_t);
+template void potrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, double *, int64_t,
+                            const double *, const double *);
+template void potrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, float *, int64_t, const float *,
+                           const float *);
 
 }  // namespace lmx
 

@@ -69,6 +69,13 @@ __device__ inline void tile_of(int tiles_m, int tiles_n, bool lower, int &tm, in
     tn = in_g / gsize;
 }
 
+// Pointer offsets of a batched launch: entry (zk, zp) starts at
+// zk * x2 + zp * x1 elements from the base pointers.
+struct BatchStrides {
+    int nb1;
+    int64_t a1, b1, c1, a2, b2, c2;
+};
+
 // Row inside a 16 x 16 accumulator block of value r held by lane group lk.
 template <class T>
 __device__ inline int acc_row(int lk, int r) {
@@ -80,7 +87,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
                                                         const T *__restrict__ A, int64_t lda,
                                                         const T *__restrict__ B, int64_t ldb,
                                                         T beta, T *__restrict__ C, int64_t ldc,
-                                                        int tiles_m, int tiles_n, int lower) {
+                                                        int tiles_m, int tiles_n, int lower, BatchStrides bs) {
     constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int MT = WM / 16, NT = WN / 16;
@@ -92,6 +99,13 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
     __shared__ T As[2][BK][BM + PAD];
     __shared__ T Bs[2][BK][BN + PAD];
 
+    if (gridDim.y > 1) {
+        // batch entry blockIdx.y = zk * nb1 + zp
+        const int zk = (int)blockIdx.y / bs.nb1, zp = (int)blockIdx.y - zk * bs.nb1;
+        A += zk * bs.a2 + zp * bs.a1;
+        B += zk * bs.b2 + zp * bs.b1;
+        C += zk * bs.c2 + zp * bs.c1;
+    }
     int tm, tn;
     tile_of(tiles_m, tiles_n, lower != 0, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
@@ -219,12 +233,13 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
 
 template <class T, int BM, int BN, int BK>
 void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, const T *A,
-                 int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0) {
+                 int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0,
+                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}) {
     int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
-    dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n), block(256);
+    dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n, (unsigned)batch), block(256);
 #define LMX_GEMM_CASE(TA_, TB_)                                                              \
     hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, BK, TA_, TB_>), grid, block, 0, s, m, n, k, \
-                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, lower)
+                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, lower, bs)
     if (!ta && !tb)
         LMX_GEMM_CASE(false, false);
     else if (!ta && tb)
@@ -253,6 +268,18 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     else
         launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
                                    C, ldc);
+}
+
+template <class T>
+void gemm_batched_impl(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda,
+                       const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int nb1, int nb2, int64_t a1,
+                       int64_t b1, int64_t c1, int64_t a2, int64_t b2, int64_t c2) {
+    if (m <= 0 || n <= 0 || nb1 <= 0 || nb2 <= 0) return;
+    if ((int64_t)nb1 * nb2 > 65535) fail_arg("gemm_batched: too many batch entries");
+    TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, 2.0 * (double)m * (double)n * (double)k * nb1 * nb2);
+    BatchStrides bs{nb1, a1, b1, c1, a2, b2, c2};
+    launch_tile<T, 64, 64, 16>(s, false, false, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc, 0,
+                               nb1 * nb2, bs);
 }
 
 template <class T>
@@ -293,6 +320,19 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
           int64_t ldc) {
     gemm_impl<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
+
+template <class T>
+void gemm_batched(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
+                  int64_t ldb, T beta, T *C, int64_t ldc, int nb1, int nb2, int64_t a1, int64_t b1, int64_t c1,
+                  int64_t a2, int64_t b2, int64_t c2) {
+    gemm_batched_impl<T>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nb1, nb2, a1, b1, c1, a2, b2, c2);
+}
+template void gemm_batched<double>(hipStream_t, int64_t, int64_t, int64_t, double, const double *, int64_t,
+                                   const double *, int64_t, double, double *, int64_t, int, int, int64_t, int64_t,
+                                   int64_t, int64_t, int64_t, int64_t);
+template void gemm_batched<float>(hipStream_t, int64_t, int64_t, int64_t, float, const float *, int64_t,
+                                  const float *, int64_t, float, float *, int64_t, int, int, int64_t, int64_t,
+                                  int64_t, int64_t, int64_t, int64_t);
 
 }  // namespace lmx
 

@@ -20,6 +20,14 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
           const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
           int64_t ldc);
 
+// Batched C_z = alpha A_z B_z + beta C_z (no transposes) for z = zk nb1 + zp,
+// zk < nb2, zp < nb1; entry z's matrices start at zk x2 + zp x1 elements
+// from A, B, C.
+template <class T>
+void gemm_batched(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
+                  int64_t ldb, T beta, T *C, int64_t ldc, int nb1, int nb2, int64_t a1, int64_t b1, int64_t c1,
+                  int64_t a2, int64_t b2, int64_t c2);
+
 // Lower triangle of C = alpha A A^T + beta C (A n x k); tiles strictly above
 // the diagonal are not launched and elements above it are not written.
 void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
@@ -38,6 +46,18 @@ template <class T>
 void trsm_right(hipStream_t s, bool lower, bool trans, bool unit, int64_t nrows, int64_t n,
                 const T *A, int64_t lda, T *B, int64_t ldb);
 
+// Explicit inverses of the diagonal blocks of op(T) (blocks of order
+// tri_block(n), stored densely in tri_inverse_elems(n) elements of W), for
+// repeated solves with tri_inverse_solve (same lower / trans flags; B n x
+// nrhs, in place).
+int tri_block(int64_t n);
+int64_t tri_inverse_elems(int64_t n);
+template <class T>
+void tri_inverse_build(hipStream_t s, bool lower, bool trans, bool unit, int64_t n, const T *A, int64_t lda, T *W);
+template <class T>
+void tri_inverse_solve(hipStream_t s, bool lower, bool trans, int64_t n, int64_t nrhs, const T *A, int64_t lda,
+                       const T *W, T *B, int64_t ldb);
+
 // LU with partial pivoting, in place; writes d_ipiv[0..n) and *d_info.
 template <class T>
 void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_info);
@@ -46,10 +66,11 @@ void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t
 template <class T>
 void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
                 int32_t *d_info);
-// Solve with getrf's factors and its permutation vector.
+// Solve with getrf's factors and its permutation vector; WL / WU, when not
+// null, are tri_inverse_build's inverses of L (unit lower) and U.
 template <class T>
 void getrs_perm(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda,
-                const int32_t *d_perm, T *B, int64_t ldb);
+                const int32_t *d_perm, T *B, int64_t ldb, const T *WL = nullptr, const T *WU = nullptr);
 // Solve with getrf's factors and swap list.
 template <class T>
 void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda,
@@ -58,8 +79,10 @@ void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda,
 // Lower Cholesky, in place; writes *d_info.
 template <class T>
 void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info);
+// WL / WU, when not null, are tri_inverse_build's inverses of L and L^T.
 template <class T>
-void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb);
+void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb,
+           const T *WL = nullptr, const T *WU = nullptr);
 
 // Set *d_info = 0.
 void clear_info(hipStream_t s, int32_t *d_info);
@@ -79,6 +102,7 @@ struct LowFactor {
     const double *A = nullptr;  // the FP64 matrix (residuals); must outlive the factor's use
     int64_t lda = 0;
     float *A32 = nullptr;       // n x n FP32 factor
+    float *WL = nullptr, *WU = nullptr;  // diagonal-block inverses of the two triangles (n > 256)
     double *anorm = nullptr;    // ||A||_inf (device)
     int32_t *ipiv = nullptr, *perm = nullptr, *dinfo = nullptr;
     int info = 0;

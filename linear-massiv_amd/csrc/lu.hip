@@ -581,7 +581,7 @@ void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t
 // Solve with a permutation vector: B <- U^{-1} L^{-1} P B.
 template <class T>
 void getrs_perm(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda, const int32_t *d_perm, T *B,
-                int64_t ldb) {
+                int64_t ldb, const T *WL, const T *WU) {
     if (n == 0 || nrhs == 0) return;
     TimedRegion tr("getrs", s);
     Scratch tmp((size_t)(n * nrhs) * sizeof(T), s);
@@ -592,6 +592,11 @@ void getrs_perm(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda
     hipLaunchKernelGGL(copy2d_kernel<T>, dim3(g), dim3(256), 0, s, tmp.as<T>(), nrhs, B, ldb, (int)n,
                        (int)nrhs);
     LMX_LAUNCH_CHECK();
+    if (WL && WU) {
+        tri_inverse_solve<T>(s, true, false, n, nrhs, LU, lda, WL, B, ldb);
+        tri_inverse_solve<T>(s, false, false, n, nrhs, LU, lda, WU, B, ldb);
+        return;
+    }
     trsm_left<T>(s, true, false, true, n, nrhs, LU, lda, B, ldb);
     trsm_left<T>(s, false, false, false, n, nrhs, LU, lda, B, ldb);
 }
@@ -617,9 +622,9 @@ template void getrs<double>(hipStream_t, int64_t, int64_t, const double *, int64
 template void getrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const int32_t *, float *,
                            int64_t);
 template void getrs_perm<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const int32_t *,
-                                 double *, int64_t);
+                                 double *, int64_t, const double *, const double *);
 template void getrs_perm<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const int32_t *, float *,
-                                int64_t);
+                                int64_t, const float *, const float *);
 
 }  // namespace lmx
 

@@ -180,18 +180,35 @@ void launch1d(hipStream_t s, int64_t elems, F launch) {
 
 }  // namespace
 
+namespace {
+
+constexpr int64_t INVERSE_MIN_N = 256;  // smaller factors solve by substitution
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+size_t inverse_bytes(int64_t n) {
+    return n > INVERSE_MIN_N ? round256((size_t)tri_inverse_elems(n) * sizeof(float)) : 0;
+}
+
+}  // namespace
+
+// Layout: A32 | anorm (256 B) | ipiv, perm, info | WL | WU.
 size_t low_factor_bytes(int64_t n) {
-    return (((size_t)n * (size_t)n * sizeof(float) + 255) / 256) * 256 + (size_t)(2 * n + 8) * sizeof(int32_t) + 256;
+    return round256((size_t)n * (size_t)n * sizeof(float)) + 256 + round256((size_t)(2 * n + 8) * sizeof(int32_t)) +
+           2 * inverse_bytes(n);
 }
 
 void low_factor_bind(LowFactor &f, void *mem) {
     char *p = static_cast<char *>(mem);
-    const size_t a32 = (((size_t)f.n * (size_t)f.n * sizeof(float) + 255) / 256) * 256;
+    const size_t a32 = round256((size_t)f.n * (size_t)f.n * sizeof(float));
+    const size_t ints = round256((size_t)(2 * f.n + 8) * sizeof(int32_t));
+    const size_t w = inverse_bytes(f.n);
     f.A32 = reinterpret_cast<float *>(p);
     f.anorm = reinterpret_cast<double *>(p + a32);
     f.ipiv = reinterpret_cast<int32_t *>(p + a32 + 256);
     f.perm = f.ipiv + f.n;
     f.dinfo = f.perm + f.n;
+    f.WL = w ? reinterpret_cast<float *>(p + a32 + 256 + ints) : nullptr;
+    f.WU = w ? reinterpret_cast<float *>(p + a32 + 256 + ints + w) : nullptr;
 }
 
 void low_factor(hipStream_t s, LowFactor &f) {
@@ -202,6 +219,17 @@ void low_factor(hipStream_t s, LowFactor &f) {
         potrf<float>(s, n, f.A32, n, f.dinfo);
     else
         getrf_impl<float>(s, n, f.A32, n, f.ipiv, f.perm, f.dinfo);
+    // inverses of the factor's diagonal blocks for the refinement solves (a
+    // failed factor leaves them unused)
+    if (f.WL) {
+        if (f.spd) {
+            tri_inverse_build<float>(s, true, false, false, n, f.A32, n, f.WL);
+            tri_inverse_build<float>(s, true, true, false, n, f.A32, n, f.WU);
+        } else {
+            tri_inverse_build<float>(s, true, false, true, n, f.A32, n, f.WL);
+            tri_inverse_build<float>(s, false, false, false, n, f.A32, n, f.WU);
+        }
+    }
     hipLaunchKernelGGL(zero_d_kernel, dim3(1), dim3(256), 0, s, f.anorm, 1);
     hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, f.A, f.lda, n, f.anorm);
     LMX_LAUNCH_CHECK();
@@ -241,9 +269,9 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
         for (int it = 0; it < ITMAX; ++it) {
             launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, R, k, R32, k, n, k); });
             if (f.spd)
-                potrs<float>(s, n, k, f.A32, n, R32, k);
+                potrs<float>(s, n, k, f.A32, n, R32, k, f.WL, f.WU);
             else
-                getrs_perm<float>(s, n, k, f.A32, n, f.perm, R32, k);
+                getrs_perm<float>(s, n, k, f.A32, n, f.perm, R32, k, f.WL, f.WU);
             launch1d(s, (int64_t)nk, [&](dim3 g) {
                 hipLaunchKernelGGL(f2d_acc_kernel, g, dim3(256), 0, s, R32, k, X, ldx, n, k, it == 0 ? 1 : 0);
             });

@@ -2,29 +2,28 @@
 //
 // Every solve is written as M x = b for each right-hand-side vector x: a
 // column of B for a left-side solve (M = op(T)), a row of B for a
-// right-side solve (X op(T) = B, M = op(T)^T).  M is processed in 64 x 64
-// blocks:
-//   - many right-hand sides ("wide"): for each diagonal block in dependency
-//     order, trsm_leaf_kernel solves the block (triangle in LDS, one vector
-//     per thread) and the not-yet-solved part is updated with one MFMA GEMM;
-//   - at most 64 right-hand sides and more than one block ("chain"):
-//     trtri_blocks_kernel inverts every diagonal block in LDS (one launch),
-//     then one cooperative launch with one workgroup per block row:
-//     workgroup b streams the blocks M_bj through LDS (the next one in
-//     flight while the current one is used), waits for block j's flag,
-//     subtracts M_bj x_j on MFMA, and finally writes x_b = W_b acc and raises
-//     its own flag -- no host round trip and no per-block launch.
+// right-side solve (X op(T) = B, M = op(T)^T).
+//   - many right-hand sides ("wide"): M is processed in 64 x 64 blocks; for
+//     each diagonal block in dependency order, trsm_leaf_kernel solves the
+//     block (triangle in LDS, one vector per thread) and the not-yet-solved
+//     part is updated with one MFMA GEMM;
+//   - at most 64 right-hand sides ("block inverse"): the diagonal blocks of
+//     M, up to 1024 x 1024, are inverted explicitly (64 x 64 inverses in
+//     LDS, then doubling with batched MFMA GEMMs), and the solve is a
+//     sequence of n / 1024 block steps, each a product with the block's
+//     inverse and one update of all unsolved rows -- every launch spans
+//     the chip, and the work is one read of the triangle.  A factor that is
+//     solved with many times keeps its inverses (tri_inverse_build once,
+//     tri_inverse_solve per solve).
 #include "common.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
-#include "sync.hpp"
 
 namespace lmx {
 
 namespace {
 
 constexpr int TB = 64;
-constexpr long CHAIN_SPIN_LIMIT = 1L << 24;
 
 // Element (r, c) of M = op(T) (MT = transpose), restricted to M's triangle.
 template <class T, bool MT, bool MLOWER>
@@ -35,10 +34,13 @@ __device__ inline T m_elem(const T *A, int64_t lda, int64_t r, int64_t c) {
 }
 
 // W_b = inverse of diagonal block b of M (order min(64, n - 64 b), padded
-// to a 64 x 64 identity), stored densely at W + 4096 b.
+// to a 64 x 64 identity; the identity when the block starts at or beyond
+// n).  W holds dense wld x wld blocks (row stride wld), and the 64-blocks
+// lie along their diagonals: block b sits in wld-block b / (wld / 64) at
+// diagonal offset 64 (b % (wld / 64)).
 template <class T, bool MT, bool MLOWER, bool UNIT>
 __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const T *__restrict__ A, int64_t lda,
-                                                           T *__restrict__ W) {
+                                                           T *__restrict__ W, int wld) {
     __shared__ T Ms[TB][TB + 1];
     __shared__ T Xs[TB][TB + 1];
     __shared__ T red[4][TB];
@@ -81,8 +83,9 @@ __global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const T *__res
         }
         __syncthreads();
     }
-    T *Wb = W + (int64_t)b * TB * TB;
-    for (int e = t; e < TB * TB; e += 256) Wb[e] = Xs[e >> 6][e & 63];
+    const int per = wld / TB, zk = b / per, i = b % per;
+    T *Wb = W + (int64_t)zk * wld * wld + (int64_t)i * TB * wld + i * TB;
+    for (int e = t; e < TB * TB; e += 256) Wb[(int64_t)(e >> 6) * wld + (e & 63)] = Xs[e >> 6][e & 63];
 }
 
 // ---------------------------------------------------------------- leaf kernel
@@ -141,154 +144,115 @@ __global__ __launch_bounds__(64) void trsm_leaf_kernel(int nb, int nvec, const T
     }
 }
 
-// ---------------------------------------------------------------- chain kernel
+// ---------------------------------------------------------------- block inverses
 
-// The 64 x 64 block of M at (r0, c0) (zero outside the matrix), fetched
-// into 16 registers per thread so all loads are in flight together, and
-// stored to LDS separately.
-template <class T, bool MT>
-__device__ inline void fetch_mblock(T (&reg)[16], const T *A, int64_t lda, int n, int r0, int c0, int t) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int e = t + 256 * i, r = e >> 6, c = e & 63;
-        int64_t gr = r0 + r, gc = c0 + c;
-        reg[i] = (gr < n && gc < n) ? (MT ? A[gc * lda + gr] : A[gr * lda + gc]) : T(0);
-    }
-}
-template <class T, int LD>
-__device__ inline void store_mblock(T (*Ms)[LD], const T (&reg)[16], int t) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int e = t + 256 * i;
-        Ms[e >> 6][e & 63] = reg[i];
-    }
-}
+// Largest diagonal block inverted explicitly, and the longest row the GEMV
+// kernel takes.
+constexpr int IBW = 1024;
 
-// Visibility between workgroups: a block's solution is written to B with
-// device-coherent stores (xstore), every wave drains them, and after a
-// barrier thread 0 raises the flag; the reader's thread 0 spins on the flag,
-// and after the barrier the block is read with device-coherent loads.
+// W (nbw dense blocks of bw x bw, row stride bw) = the diagonal bw-blocks of
+// M = op(T) restricted to M's strict triangle, zero elsewhere and beyond n.
+// One workgroup per 64 x 64 tile (blockIdx.y = diagonal block); a
+// transposed operand goes through LDS so both sides stay coalesced.
 template <class T, bool MT, bool MLOWER>
-__global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int nrhs, const T *__restrict__ A, int64_t lda,
-                                                         const T *__restrict__ W, T *B, int64_t ldb,
-                                                         int *flags, int *err) {
-    constexpr int PAD = sizeof(T) == 8 ? 1 : 2;
-    constexpr int LD = TB + PAD;
-    __shared__ T Ms[2][TB][LD];
-    __shared__ T Ws[TB][LD];
-    __shared__ T Xs[TB][LD];
-    typedef typename V4<T>::type V;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, lr = lane & 15, lk = lane >> 4;
-    const int nblk = gridDim.x;
-    const int b = MLOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
-    const int r0 = b * TB;
-    const int ncb = (nrhs + 15) >> 4;  // 16-column blocks (<= 4)
-    const int ndep = MLOWER ? b : nblk - 1 - b;
-
-    // W_b and the first off-diagonal block are fetched before any waiting.
-    const T *Wb = W + (int64_t)b * TB * TB;
-#pragma unroll
-    for (int e = t; e < TB * TB; e += 256) Ws[e >> 6][e & 63] = Wb[e];
-    T nxt[16];
-    if (ndep > 0) {
-        fetch_mblock<T, MT>(nxt, A, lda, n, r0, (MLOWER ? 0 : nblk - 1) * TB, t);
-        store_mblock<T, LD>(Ms[0], nxt, t);
-    }
-
-    V acc[4];
-    // acc <- B_b (rows r0 + 16 wave + acc_row, cols 16 cb + lr)
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int gi = r0 + wave * 16 + acc_row<T>(lk, r), gc = cb * 16 + lr;
-            acc[cb][r] = (cb < ncb && gi < n && gc < nrhs) ? B[(int64_t)gi * ldb + gc] : T(0);
+__global__ __launch_bounds__(256) void tri_diag_copy_kernel(int n, int bw, const T *__restrict__ A, int64_t lda,
+                                                            T *__restrict__ W) {
+    __shared__ T tile[TB][TB + 1];
+    const int tpb = bw / TB, t = threadIdx.x;
+    const int ti = (int)blockIdx.x / tpb, tj = (int)blockIdx.x % tpb;
+    const int64_t r0 = (int64_t)blockIdx.y * bw + ti * TB, c0 = (int64_t)blockIdx.y * bw + tj * TB;
+    T *Wt = W + (int64_t)blockIdx.y * bw * bw + (int64_t)ti * TB * bw + tj * TB;
+    const bool live = MLOWER ? ti >= tj : ti <= tj;  // uniform per workgroup
+    if (live) {
+#pragma unroll 4
+        for (int e = t; e < TB * TB; e += 256) {
+            const int a = e >> 6, b = e & 63;
+            // (a, b) in the stored operand's memory order
+            const int64_t gr = MT ? c0 + a : r0 + a, gc = MT ? r0 + b : c0 + b;
+            const T v = (gr < n && gc < n) ? A[gr * lda + gc] : T(0);
+            if (MT)
+                tile[b][a] = v;
+            else
+                tile[a][b] = v;
         }
-    }
-    bool failed = false;
-    for (int d = 0; d < ndep; ++d) {
-        const int j = MLOWER ? d : nblk - 1 - d;
-        const int c0 = j * TB;
-        const int cur = d & 1;
-        if (t == 0 && !failed) {
-            long spins = 0;
-            while (xload(&flags[j]) == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > CHAIN_SPIN_LIMIT) {
-                    xstore(err, 1);
-                    failed = true;
-                    break;
-                }
-            }
-        }
-        __syncthreads();  // Ms[cur] loaded, flag j seen, Xs free
-#pragma unroll
-        for (int i = 0; i < TB * TB / 256; ++i) {
-            const int e = t + 256 * i;
-            if (e < TB * nrhs) {
-                int r = e / nrhs, c = e - r * nrhs;
-                int64_t gr = c0 + r;
-                Xs[r][c] = gr < n ? xload(&B[gr * ldb + c]) : T(0);
-            }
-        }
-        // the next block's loads fly while this block is used; they land in
-        // the other LDS buffer after the MFMAs
-        const bool more = d + 1 < ndep;
-        if (more) fetch_mblock<T, MT>(nxt, A, lda, n, r0, (MLOWER ? j + 1 : j - 1) * TB, t);
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < TB; kk += 4) {
-            T a = -Ms[cur][wave * 16 + lr][kk + lk];
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-                if (cb < ncb) acc[cb] = mfma(a, Xs[kk + lk][cb * 16 + lr], acc[cb]);
-        }
-        if (more) store_mblock<T, LD>(Ms[cur ^ 1], nxt, t);
     }
-    // x_b = W_b acc
-    __syncthreads();
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Xs[wave * 16 + acc_row<T>(lk, r)][cb * 16 + lr] = acc[cb][r];
-    __syncthreads();
-    V out[4];
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) out[cb] = V{0, 0, 0, 0};
 #pragma unroll 4
-    for (int kk = 0; kk < TB; kk += 4) {
-        T a = Ws[wave * 16 + lr][kk + lk];
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-            if (cb < ncb) out[cb] = mfma(a, Xs[kk + lk][cb * 16 + lr], out[cb]);
+    for (int e = t; e < TB * TB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        const int64_t gr = r0 + r, gc = c0 + c;
+        const bool in = live && (MLOWER ? gc < gr : gc > gr);
+        Wt[(int64_t)r * bw + c] = in ? tile[r][c] : T(0);
     }
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int gi = r0 + wave * 16 + acc_row<T>(lk, r), gc = cb * 16 + lr;
-            if (cb < ncb && gi < n && gc < nrhs) xstore(&B[(int64_t)gi * ldb + gc], out[cb][r]);
-        }
-    xdrain();
-    __syncthreads();
-    if (t == 0) xstore(&flags[b], 1);
 }
 
-__global__ void zero_i32_kernel(int *p, int n) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0;
+// out[r][q] = alpha sum_{c<k} M[r][c] x[c][q] (added to out[r][q] when
+// `accumulate`), r < rows, q < nrhs <= NR, k <= IBW.  One wave per row, rpw
+// rows per wave; x is staged in LDS column by column (consecutive lanes read
+// consecutive banks) and a lane's 16 row elements are all loaded before the
+// products, so a wave has its whole row in flight.
+template <class T, int NR>
+__global__ __launch_bounds__(256) void gemv_rows_kernel(int rows, int k, int nrhs, int rpw, T alpha,
+                                                        const T *__restrict__ M, int64_t ldm,
+                                                        const T *__restrict__ x, int64_t ldx, int accumulate,
+                                                        T *__restrict__ out, int64_t ldo) {
+    __shared__ T xs[NR][IBW];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int e = t; e < NR * IBW; e += 256) {
+        const int c = e / NR, q = e % NR;
+        xs[q][c] = (c < k && q < nrhs) ? x[(int64_t)c * ldx + q] : T(0);
+    }
+    __syncthreads();
+    const int64_t rbeg = ((int64_t)blockIdx.x * 4 + wave) * rpw;
+    for (int rr = 0; rr < rpw; ++rr) {
+        const int64_t r = rbeg + rr;
+        if (r >= rows) break;  // wave-uniform
+        const T *mr = M + r * ldm;
+        T mv[IBW / 64];
+#pragma unroll
+        for (int i = 0; i < IBW / 64; ++i) {
+            const int c = lane + 64 * i;
+            mv[i] = c < k ? mr[c] : T(0);
+        }
+        T acc[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = T(0);
+#pragma unroll
+        for (int i = 0; i < IBW / 64; ++i)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = fma(mv[i], xs[q][lane + 64 * i], acc[q]);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const T v = wave_sum(acc[q]);
+            if (lane == q && q < nrhs) {
+                T *o = out + r * ldo + q;
+                *o = accumulate ? fma(alpha, v, *o) : alpha * v;
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ void copy_rows_kernel(const T *__restrict__ in, int64_t ldi, T *__restrict__ out, int64_t ldo,
+                                 int64_t rows, int cols) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * cols) return;
+    const int64_t i = e / cols;
+    const int c = (int)(e - i * cols);
+    out[i * ldo + c] = in[i * ldi + c];
 }
 
 // ---------------------------------------------------------------- dispatch
 
 template <class T>
-void trtri_blocks(hipStream_t s, bool mt, bool mlower, bool unit, int n, const T *A, int64_t lda, T *W) {
-    dim3 g((unsigned)cdiv(n, TB)), blk(256);
-#define LMX_TRTRI(MT_, ML_, U_)                                                                 \
-    if (mt == MT_ && mlower == ML_ && unit == U_) {                                            \
-        hipLaunchKernelGGL((trtri_blocks_kernel<T, MT_, ML_, U_>), g, blk, 0, s, n, A, lda, W); \
-        LMX_LAUNCH_CHECK();                                                                     \
-        return;                                                                                 \
+void trtri_blocks(hipStream_t s, bool mt, bool mlower, bool unit, int n, int nblocks, const T *A, int64_t lda, T *W,
+                  int wld) {
+    dim3 g((unsigned)nblocks), blk(256);
+#define LMX_TRTRI(MT_, ML_, U_)                                                                      \
+    if (mt == MT_ && mlower == ML_ && unit == U_) {                                                 \
+        hipLaunchKernelGGL((trtri_blocks_kernel<T, MT_, ML_, U_>), g, blk, 0, s, n, A, lda, W, wld); \
+        LMX_LAUNCH_CHECK();                                                                          \
+        return;                                                                                      \
     }
     LMX_TRTRI(false, false, false)
     LMX_TRTRI(false, false, true)
@@ -302,35 +266,48 @@ void trtri_blocks(hipStream_t s, bool mt, bool mlower, bool unit, int n, const T
 }
 
 template <class T>
-int chain_max_blocks() {
-    static int cached = -1;
-    if (cached < 0) {
-        int dev = 0, per_cu = 0;
-        LMX_HIP(hipGetDevice(&dev));
-        LMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trsm_chain_kernel<T, false, true>, 256, 0));
-        cached = std::max(per_cu, 0) * ctx().cu_count;
+void diag_copy(hipStream_t s, bool mt, bool mlower, int n, int bw, int nbw, const T *A, int64_t lda, T *W) {
+    const int tpb = bw / TB;
+    dim3 g((unsigned)(tpb * tpb), (unsigned)nbw), blk(256);
+#define LMX_DCOPY(MT_, ML_)                                                                             \
+    if (mt == MT_ && mlower == ML_) {                                                                  \
+        hipLaunchKernelGGL((tri_diag_copy_kernel<T, MT_, ML_>), g, blk, 0, s, n, bw, A, lda, W);        \
+        LMX_LAUNCH_CHECK();                                                                             \
+        return;                                                                                         \
     }
-    return cached;
+    LMX_DCOPY(false, false)
+    LMX_DCOPY(false, true)
+    LMX_DCOPY(true, false)
+    LMX_DCOPY(true, true)
+#undef LMX_DCOPY
 }
 
-// Chain solve for nrhs <= 64; returns false when the grid cannot be co-resident.
+// out = alpha op(M) x (+ out when `accumulate`), op(M) rows x k, x k x nrhs:
+// the GEMV kernel for at most 8 right-hand sides of a row-major M, the MFMA
+// GEMM otherwise.
 template <class T>
-bool chain_solve(hipStream_t s, bool mt, bool mlower, int n, int nrhs, const T *A, int64_t lda, const T *W, T *B,
-                 int64_t ldb) {
-    int nblk = (int)cdiv(n, TB);
-    if (nblk > chain_max_blocks<T>()) return false;
-    Scratch fl((size_t)(nblk + 64) * sizeof(int), s);
-    int *flags = fl.as<int>(), *err = flags + nblk;
-    hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)cdiv(nblk + 1, 256)), dim3(256), 0, s, flags, nblk + 1);
+void apply_block(hipStream_t s, int64_t rows, int64_t k, int64_t nrhs, T alpha, const T *M, int64_t ldm, bool mtrans,
+                 const T *x, int64_t ldx, bool accumulate, T *out, int64_t ldo) {
+    if (rows <= 0) return;
+    if (mtrans || nrhs > 8) {
+        gemm(s, mtrans, false, rows, nrhs, k, alpha, M, ldm, x, ldx, accumulate ? T(1) : T(0), out, ldo);
+        return;
+    }
+    const int rpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, rows / 4096));
+    dim3 g((unsigned)cdiv(rows, 4 * rpw)), blk(256);
+#define LMX_GEMV(NR_)                                                                                           \
+    hipLaunchKernelGGL((gemv_rows_kernel<T, NR_>), g, blk, 0, s, (int)rows, (int)k, (int)nrhs, rpw, alpha, M, ldm, \
+                       x, ldx, accumulate ? 1 : 0, out, ldo)
+    if (nrhs == 1)
+        LMX_GEMV(1);
+    else if (nrhs == 2)
+        LMX_GEMV(2);
+    else if (nrhs <= 4)
+        LMX_GEMV(4);
+    else
+        LMX_GEMV(8);
+#undef LMX_GEMV
     LMX_LAUNCH_CHECK();
-    void *args[] = {&n, &nrhs, &A, &lda, &W, &B, &ldb, &flags, &err};
-    const void *fn = nullptr;
-    if (!mt && mlower) fn = (const void *)trsm_chain_kernel<T, false, true>;
-    if (!mt && !mlower) fn = (const void *)trsm_chain_kernel<T, false, false>;
-    if (mt && mlower) fn = (const void *)trsm_chain_kernel<T, true, true>;
-    if (mt && !mlower) fn = (const void *)trsm_chain_kernel<T, true, false>;
-    LMX_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(256), args, 0, s));
-    return true;
 }
 
 // Leaf solve of one diagonal block (order bs) for nvec vectors.
@@ -358,6 +335,78 @@ void leaf(hipStream_t s, bool mt, bool mlower, bool unit, int bs, int64_t nvec, 
 
 }  // namespace
 
+int tri_block(int64_t n) {
+    int b = TB;
+    while (b < IBW && b < n) b *= 2;
+    return b;
+}
+
+int64_t tri_inverse_elems(int64_t n) {
+    const int bw = tri_block(n);
+    return cdiv(std::max<int64_t>(n, 1), bw) * (int64_t)bw * bw;
+}
+
+// Inverses of M's diagonal bw-blocks: masked copy, the 64 x 64 inverses in
+// place on its diagonal, then doubling: for each pair of inverted h-blocks
+// of a lower M = [[A, 0], [C, B]] the off-diagonal block of the inverse is
+// -B^-1 (C A^-1); of an upper M = [[A, C], [0, B]] it is -A^-1 (C B^-1).
+// Each level is two batched GEMMs over every pair of every diagonal block.
+template <class T>
+void tri_inverse_build(hipStream_t s, bool lower, bool trans, bool unit, int64_t n, const T *A, int64_t lda, T *W) {
+    if (n <= 0) return;
+    if (n > INT32_MAX / 2) fail_arg("trsm: dimension too large");
+    TimedRegion tr("trsm", s);
+    const bool mt = trans, mlower = lower != trans;
+    const int bw = tri_block(n), nbw = (int)cdiv(n, bw), tpb = bw / TB;
+    diag_copy<T>(s, mt, mlower, (int)n, bw, nbw, A, lda, W);
+    trtri_blocks<T>(s, mt, mlower, unit, (int)n, nbw * tpb, A, lda, W, bw);
+    if (bw == TB) return;
+    Scratch sT((size_t)nbw * bw * (bw / 4) * sizeof(T), s);
+    T *Tb = sT.as<T>();
+    const int64_t bw2 = (int64_t)bw * bw;
+    for (int h = TB; h < bw; h *= 2) {
+        const int np = bw / (2 * h);
+        const int64_t dstep = 2 * (int64_t)h * bw + 2 * h, hh = (int64_t)h * h;
+        T *Cp = mlower ? W + (int64_t)h * bw : W + h;        // off-diagonal block of each pair
+        const T *Rinv = mlower ? W : W + (int64_t)h * bw + h;  // inverse applied on C's right
+        const T *Linv = mlower ? W + (int64_t)h * bw + h : W;  // inverse applied on the left
+        gemm_batched<T>(s, h, h, h, T(1), Cp, bw, Rinv, bw, T(0), Tb, h, np, nbw, dstep, dstep, hh, bw2, bw2,
+                        np * hh);
+        gemm_batched<T>(s, h, h, h, T(-1), Linv, bw, Tb, h, T(0), Cp, bw, np, nbw, dstep, hh, dstep, bw2, np * hh,
+                        bw2);
+    }
+}
+
+// op(T) X = B with the inverses of tri_inverse_build: for each diagonal
+// block in dependency order, X_b = W_b B_b, then the rows still unsolved
+// take B_i -= M_ib X_b -- two launches per block, each a full-chip pass.
+template <class T>
+void tri_inverse_solve(hipStream_t s, bool lower, bool trans, int64_t n, int64_t nrhs, const T *A, int64_t lda,
+                       const T *W, T *B, int64_t ldb) {
+    if (n <= 0 || nrhs <= 0) return;
+    if (n > INT32_MAX / 2 || nrhs > INT32_MAX / 2) fail_arg("trsm: dimension too large");
+    TimedRegion tr("trsm", s, (double)n * (double)n * (double)nrhs);
+    const bool mt = trans, mlower = lower != trans;
+    const int bw = tri_block(n), nbw = (int)cdiv(n, bw);
+    Scratch sX((size_t)n * nrhs * sizeof(T), s);
+    T *X = sX.as<T>();
+    const int64_t ldx = nrhs;
+    for (int bi = 0; bi < nbw; ++bi) {
+        const int b = mlower ? bi : nbw - 1 - bi;
+        const int64_t j0 = (int64_t)b * bw, nj = std::min<int64_t>(bw, n - j0);
+        apply_block<T>(s, nj, nj, nrhs, T(1), W + (int64_t)b * bw * bw, bw, false, B + j0 * ldb, ldb, false,
+                       X + j0 * ldx, ldx);
+        const int64_t i0 = mlower ? j0 + nj : 0, ni = mlower ? n - (j0 + nj) : j0;
+        if (ni > 0) {
+            const T *Mb = mt ? A + j0 * lda + i0 : A + i0 * lda + j0;
+            apply_block<T>(s, ni, nj, nrhs, T(-1), Mb, lda, mt, X + j0 * ldx, ldx, true, B + i0 * ldb, ldb);
+        }
+    }
+    hipLaunchKernelGGL(copy_rows_kernel<T>, dim3((unsigned)cdiv(n * nrhs, 256)), dim3(256), 0, s, X, ldx, B, ldb, n,
+                       (int)nrhs);
+    LMX_LAUNCH_CHECK();
+}
+
 // op(T) X = B, B n x nrhs.
 template <class T>
 void trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int64_t n, int64_t nrhs, const T *A, int64_t lda,
@@ -367,11 +416,11 @@ void trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int64_t n, int6
     TimedRegion tr("trsm", s, (double)n * (double)n * (double)nrhs);
     const bool mt = trans, mlower = lower != trans;
     const int64_t nblk = cdiv(n, TB);
-    if (nrhs <= 64 && nblk > 1) {
-        Scratch sw((size_t)nblk * TB * TB * sizeof(T), s);
-        T *W = sw.as<T>();
-        trtri_blocks<T>(s, mt, mlower, unit, (int)n, A, lda, W);
-        if (chain_solve<T>(s, mt, mlower, (int)n, (int)nrhs, A, lda, W, B, ldb)) return;
+    if (nrhs <= 64 && n > 4 * TB) {
+        Scratch sw((size_t)tri_inverse_elems(n) * sizeof(T), s);
+        tri_inverse_build<T>(s, lower, trans, unit, n, A, lda, sw.as<T>());
+        tri_inverse_solve<T>(s, lower, trans, n, nrhs, A, lda, sw.as<T>(), B, ldb);
+        return;
     }
     // M[r0.., c0..] as a GEMM operand: the T block, transposed when trans.
     auto mblock = [&](int64_t r0, int64_t c0) -> const T * { return trans ? A + c0 * lda + r0 : A + r0 * lda + c0; };
@@ -418,6 +467,12 @@ template void trsm_left<double>(hipStream_t, bool, bool, bool, int64_t, int64_t,
                                 int64_t);
 template void trsm_left<float>(hipStream_t, bool, bool, bool, int64_t, int64_t, const float *, int64_t, float *,
                                int64_t);
+template void tri_inverse_build<double>(hipStream_t, bool, bool, bool, int64_t, const double *, int64_t, double *);
+template void tri_inverse_build<float>(hipStream_t, bool, bool, bool, int64_t, const float *, int64_t, float *);
+template void tri_inverse_solve<double>(hipStream_t, bool, bool, int64_t, int64_t, const double *, int64_t,
+                                        const double *, double *, int64_t);
+template void tri_inverse_solve<float>(hipStream_t, bool, bool, int64_t, int64_t, const float *, int64_t,
+                                       const float *, float *, int64_t);
 template void trsm_right<double>(hipStream_t, bool, bool, bool, int64_t, int64_t, const double *, int64_t, double *,
                                  int64_t);
 template void trsm_right<float>(hipStream_t, bool, bool, bool, int64_t, int64_t, const float *, int64_t, float *,

@@ -97,7 +97,7 @@ def test_device_posv_large(lmx_gpu, n):
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("diag", ["N", "U"])
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 3), (64, 1), (65, 2), (200, 70)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 3), (64, 1), (65, 2), (200, 70), (300, 1), (1100, 3), (2100, 9)])
 def test_trsm_vs_scipy(lmx_gpu, uplo, trans, diag, n, nrhs):
     rng = np.random.default_rng(n * 7 + nrhs)
     T = rng.uniform(-1, 1, (n, n)) / np.sqrt(n) + np.diag(rng.uniform(1, 2, n))
