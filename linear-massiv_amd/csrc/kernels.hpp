@@ -62,10 +62,12 @@ void tri_inverse_solve(hipStream_t s, bool lower, bool trans, int64_t n, int64_t
 template <class T>
 void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_info);
 // getrf that also writes the permutation vector (row i of PA is row
-// d_perm[i] of A) when d_perm is not null.
+// d_perm[i] of A) when d_perm is not null.  `tournament` chooses each
+// panel's pivots by a tournament over the panel's row blocks (CALU) instead
+// of column-by-column partial pivoting: one exchange per panel.
 template <class T>
 void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
-                int32_t *d_info);
+                int32_t *d_info, bool tournament = false);
 // Solve with getrf's factors and its permutation vector; WL / WU, when not
 // null, are tri_inverse_build's inverses of L (unit lower) and U.
 template <class T>

@@ -381,15 +381,296 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
 }
 
 
+// ---------------------------------------------------------------- tournament panel
+//
+// Panel with one exchange per panel instead of one per column (for the FP32
+// factor of the mixed solver, which only has to precondition the FP64
+// refinement):
+//   tslu_local_kernel: every workgroup runs GEPP on its own rows only (no
+//     row moves; chosen rows are marked) and writes the kb rows it chose,
+//     with their original values, as candidates;
+//   tslu_final_kernel: every workgroup runs GEPP on the P*kb candidates --
+//     identical work in each, so no cross-workgroup wait -- which gives the
+//     panel's pivot rows in order with their factored values (L11 \ U11),
+//     turns the pivot order into LAPACK's sequential swaps, and computes the
+//     multipliers of its own rows by the update sequence GEPP applies to a
+//     row that is never chosen: l_c = (a_c - sum_{k<c} l_k u_kc) / u_cc.
+// The result is the LU factorisation of the panel with the tournament's
+// pivots: GEPP itself when the panel fits one workgroup, in general the
+// row choice of CALU with a flat reduction tree (Grigori, Demmel and Xiang,
+// SIAM J. Matrix Anal. Appl. 32(4), 2011).  Ties go to the lowest original
+// row, NaN is chosen only when nothing else is left.
+
+template <class T, int NR>
+struct GeppRows {
+    T v[NR][KB];
+    int id[NR];   // panel-relative row, -1 = none
+    int col[NR];  // column at which the row became a pivot, -1 = not yet
+};
+
+template <class T>
+struct GeppLds {
+    T slot_data[2][PW][KB];
+    T slot_val[2][PW];
+    int slot_id[2][PW];
+    int piv_id[KB];
+};
+
+// Column J of GEPP over the rows the workgroup holds.
+template <class T, int NR, int J>
+__device__ __forceinline__ void gepp_column(GeppRows<T, NR> &st, GeppLds<T> &L) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, par = J & 1;
+    T a[NR];
+    T tmax = T(-1);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const T x = fabs(st.v[q][J]);
+        a[q] = (st.id[q] >= 0 && st.col[q] < 0) ? (isnan(x) ? T(-0.5) : x) : T(-1);
+        tmax = fmax(tmax, a[q]);
+    }
+    const T wmax = wave_max(tmax);
+    int tid = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        if (a[q] == wmax && a[q] > T(-1)) tid = min(tid, st.id[q]);
+    const int wid = wave_min(tid);
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        if (st.id[q] == wid && st.col[q] < 0) {
+#pragma unroll
+            for (int c = J; c < KB; ++c) L.slot_data[par][wave][c] = st.v[q][c];
+        }
+    if (lane == 0) {
+        L.slot_val[par][wave] = wmax;
+        L.slot_id[par][wave] = wid;
+    }
+    __syncthreads();
+    T best = T(-1);
+    int bid = INT_MAX, bw = -1;
+#pragma unroll
+    for (int x = 0; x < PW; ++x) {
+        const T vx = L.slot_val[par][x];
+        const int ix = L.slot_id[par][x];
+        if (vx > best || (vx == best && ix < bid)) {
+            best = vx;
+            bid = ix;
+            bw = x;
+        }
+    }
+    if (bw < 0 || bid == INT_MAX) {  // no row left (uniform)
+        if (t == 0) L.piv_id[J] = -1;
+        return;
+    }
+    T pr[KB];
+#pragma unroll
+    for (int c = J; c < KB; ++c) pr[c] = L.slot_data[par][bw][c];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        if (st.id[q] == bid && st.col[q] < 0) st.col[q] = J;
+    if (t == 0) L.piv_id[J] = bid;
+    const T piv = pr[J];
+    if (piv != T(0)) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            if (st.id[q] >= 0 && st.col[q] < 0) {
+                const T l = st.v[q][J] / piv;
+                st.v[q][J] = l;
+#pragma unroll
+                for (int c = J + 1; c < KB; ++c) st.v[q][c] = fma(-l, pr[c], st.v[q][c]);
+            }
+    }
+}
+
+template <class T, int NR, int J>
+__device__ __forceinline__ void gepp_from(GeppRows<T, NR> &st, GeppLds<T> &L, int kb) {
+    if constexpr (J < KB) {
+        if (J < kb) {
+            gepp_column<T, NR, J>(st, L);
+            gepp_from<T, NR, J + 1>(st, L, kb);
+        }
+    }
+}
+
+// Scratch of the tournament panel: candidates of up to MAXP workgroups and
+// the panel's first kb rows (original values).
+template <class T>
+struct TsluScratch {
+    int32_t cand[MAXP * KB];
+    T candv[MAXP * KB * KB];
+    T top[KB * KB];
+};
+
+template <class T>
+__global__ __launch_bounds__(PT) void tslu_local_kernel(int m, int kb, const T *__restrict__ A, int64_t lda,
+                                                         TsluScratch<T> *__restrict__ ts) {
+    const int w = blockIdx.x, t = threadIdx.x;
+    __shared__ GeppLds<T> L;
+    GeppRows<T, RPT> st;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = w * PANEL_ROWS + q * PT + t;
+        st.id[q] = r < m ? r : -1;
+        st.col[q] = -1;
+#pragma unroll
+        for (int c = 0; c < KB; ++c) st.v[q][c] = (r < m && c < kb) ? A[(int64_t)r * lda + c] : T(0);
+    }
+    gepp_from<T, RPT, 0>(st, L, kb);
+    __syncthreads();
+    for (int e = t; e < KB * KB; e += PT) {
+        const int i = e / KB, c = e % KB;
+        const int r = i < kb ? L.piv_id[i] : -1;
+        if (c == 0) ts->cand[w * KB + i] = r;
+        ts->candv[(w * KB + i) * KB + c] = (r >= 0 && c < kb) ? A[(int64_t)r * lda + c] : T(0);
+    }
+    if (w == 0)
+        for (int e = t; e < KB * KB; e += PT) {
+            const int i = e / KB, c = e % KB;
+            ts->top[e] = (i < m && c < kb) ? A[(int64_t)i * lda + c] : T(0);
+        }
+}
+
+template <class T, int NC>
+__global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__restrict__ A, int64_t lda, int k0,
+                                                         int P, const TsluScratch<T> *__restrict__ ts,
+                                                         int32_t *__restrict__ ipiv, int32_t *__restrict__ srcmap,
+                                                         int32_t *__restrict__ moves, int32_t *__restrict__ info) {
+    const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    __shared__ GeppLds<T> L;
+    __shared__ T F[KB][KB + 1];  // the pivot rows, factored: L11 \ U11
+    __shared__ int ent_pos[2 * KB], ent_row[2 * KB];
+    __shared__ int s_nent;
+    GeppRows<T, NC> st;
+    for (int e = t; e < KB * (KB + 1); e += PT) (&F[0][0])[e] = T(0);
+    const int ncand = P * KB;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const int ci = q * PT + t;
+        const int r = ci < ncand ? ts->cand[ci] : -1;
+        st.id[q] = r;
+        st.col[q] = -1;
+#pragma unroll
+        for (int c = 0; c < KB; ++c) st.v[q][c] = r >= 0 ? ts->candv[ci * KB + c] : T(0);
+    }
+    gepp_from<T, NC, 0>(st, L, kb);
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+        if (st.col[q] >= 0) {
+#pragma unroll
+            for (int c = 0; c < KB; ++c) F[st.col[q]][c] = st.v[q][c];
+        }
+    __syncthreads();
+    // pivot order -> sequential swaps of positions j and cur(p_j); lane e of
+    // wave 0 holds entry e of the (position, row) list of moved positions
+    if (wave == 0) {
+        int epos = -1, erow = -1, nent = 0;
+        for (int j = 0; j < kb; ++j) {
+            const int r = L.piv_id[j];
+            const unsigned long long m1 = __ballot(erow == r);
+            const int cur = m1 ? __shfl(epos, __ffsll((long long)m1) - 1) : r;
+            const unsigned long long m2 = __ballot(epos == j);
+            const int rj = m2 ? __shfl(erow, __ffsll((long long)m2) - 1) : j;
+            if (m2) {
+                if (lane == __ffsll((long long)m2) - 1) erow = r;
+            } else {
+                if (lane == nent) {
+                    epos = j;
+                    erow = r;
+                }
+                ++nent;
+            }
+            if (cur != j) {
+                const unsigned long long m3 = __ballot(epos == cur);
+                if (m3) {
+                    if (lane == __ffsll((long long)m3) - 1) erow = rj;
+                } else {
+                    if (lane == nent) {
+                        epos = cur;
+                        erow = rj;
+                    }
+                    ++nent;
+                }
+            }
+            if (w == 0 && lane == 0) ipiv[k0 + j] = k0 + cur;
+        }
+        ent_pos[lane] = epos;
+        ent_row[lane] = erow;
+        if (lane == 0) s_nent = nent;
+        if (w == 0) {
+            if (epos >= 0) {
+                srcmap[epos] = erow;
+                moves[1 + lane] = epos;
+            }
+            if (lane == 0) moves[0] = nent;
+            if (lane == 0) {
+                int inf = 0;
+                for (int j = 0; j < kb; ++j)
+                    if (F[j][j] == T(0)) {
+                        inf = k0 + j + 1;
+                        break;
+                    }
+                if (inf != 0 && *info == 0) *info = inf;
+            }
+        }
+    }
+    __syncthreads();
+    const int nent = s_nent;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int pos = w * PANEL_ROWS + q * PT + t;
+        if (pos >= m) continue;
+        T *row = A + (int64_t)pos * lda;
+        if (pos < kb) {
+#pragma unroll
+            for (int c = 0; c < KB; ++c)
+                if (c < kb) row[c] = F[pos][c];
+            continue;
+        }
+        int src = pos;  // a displaced top row, or the row itself
+        for (int e = 0; e < nent; ++e)
+            if (ent_pos[e] == pos) src = ent_row[e];
+        T x[KB];
+        const bool own = src == pos || src < 0 || src >= KB;  // displaced rows come from the first kb
+#pragma unroll
+        for (int c = 0; c < KB; ++c) x[c] = c < kb ? (own ? row[c] : ts->top[src * KB + c]) : T(0);
+#pragma unroll
+        for (int c = 0; c < KB; ++c) {
+            const T u = F[c][c];
+            if (c < kb && u != T(0)) {
+                const T l = x[c] / u;
+                x[c] = l;
+#pragma unroll
+                for (int k = c + 1; k < KB; ++k) x[k] = fma(-l, F[c][k], x[k]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < KB; ++c)
+            if (c < kb) row[c] = x[c];
+    }
+}
+
 // X[d] <- X[src[d]] for every d of the move list, for columns [c0, c1) and
-// [c2, c3) of X (rows relative to X).  All reads of a column finish before
-// any write (one workgroup per 64 columns, barrier in between).
+// [c2, c3) of X (rows relative to X), and the same gather on the
+// permutation vector when perm is not null (workgroup 0).  The move list
+// is read once into LDS; all reads of a column finish before any write (one
+// workgroup per 64 columns, barrier in between).
 template <class T>
 __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
                                                          int c2, int c3, const int32_t *__restrict__ moves,
-                                                         const int32_t *__restrict__ srcmap) {
-    constexpr int PER = (2 * KB + 3) / 4;  // moves per thread
+                                                         const int32_t *__restrict__ srcmap,
+                                                         int32_t *__restrict__ perm) {
+    constexpr int MAXM = 2 * KB;
+    constexpr int PER = MAXM / 4;  // moves per thread
+    __shared__ int s_dst[MAXM], s_src[MAXM], s_cnt;
     const int t = threadIdx.x, cl = t & 63, grp = t >> 6;
+    if (t < 64) {
+        const int cnt = min(moves[0], MAXM);
+        const int d = t < cnt ? moves[1 + t] : 0;
+        s_dst[t] = d;
+        s_src[t] = t < cnt ? srcmap[d] : 0;
+        if (t == 0) s_cnt = cnt;
+    }
+    __syncthreads();
+    const int cnt = s_cnt;
     const int n1 = c1 - c0;
     const int cc = blockIdx.x * 64 + cl;
     int col = -1;
@@ -397,33 +678,23 @@ __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int6
         col = c0 + cc;
     else if (cc - n1 < c3 - c2)
         col = c2 + (cc - n1);
-    const int cnt = moves[0];
     T val[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        int e = grp * PER + i;
+        const int e = grp * PER + i;
         val[i] = T(0);
-        if (col >= 0 && e < cnt) val[i] = X[(int64_t)srcmap[moves[1 + e]] * ldx + col];
+        if (col >= 0 && e < cnt) val[i] = X[(int64_t)s_src[e] * ldx + col];
     }
+    int pv = 0;
+    const bool pm = perm != nullptr && blockIdx.x == 0 && t < cnt;
+    if (pm) pv = perm[s_src[t]];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        int e = grp * PER + i;
-        if (col >= 0 && e < cnt) X[(int64_t)moves[1 + e] * ldx + col] = val[i];
+        const int e = grp * PER + i;
+        if (col >= 0 && e < cnt) X[(int64_t)s_dst[e] * ldx + col] = val[i];
     }
-}
-
-// perm[d] <- perm[src[d]] over the move list (one workgroup).
-__global__ __launch_bounds__(64) void perm_gather_kernel(int32_t *__restrict__ perm,
-                                                         const int32_t *__restrict__ moves,
-                                                         const int32_t *__restrict__ srcmap) {
-    const int t = threadIdx.x, cnt = moves[0];
-    int a = 0, b = 0;
-    if (t < cnt) a = perm[srcmap[moves[1 + t]]];
-    if (t + 64 < cnt) b = perm[srcmap[moves[1 + t + 64]]];
-    __syncthreads();
-    if (t < cnt) perm[moves[1 + t]] = a;
-    if (t + 64 < cnt) perm[moves[1 + t + 64]] = b;
+    if (pm) perm[s_dst[t]] = pv;
 }
 
 __global__ void iota_kernel(int32_t *p, int n) {
@@ -499,14 +770,37 @@ void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32
     base += (unsigned)kb;  // sequence numbers stay monotonic across launches
 }
 
+// The tournament panel of an m x kb panel (same outputs as launch_panel).
+template <class T>
+void launch_tslu(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32_t *ipiv, int32_t *srcmap,
+                 int32_t *moves, int32_t *info, TsluScratch<T> *ts) {
+    const int P = (int)cdiv(m, PANEL_ROWS);
+    if (P > MAXP) fail_arg("getrf: matrix too tall for the panel kernel");
+    TimedRegion tr("getrf_panel", s);
+    hipLaunchKernelGGL(tslu_local_kernel<T>, dim3(P), dim3(PT), 0, s, m, kb, A, lda, ts);
+    LMX_LAUNCH_CHECK();
+    const int nc = (int)cdiv((int64_t)P * KB, PT);
+#define LMX_TSLU(NC_)                                                                                              \
+    hipLaunchKernelGGL((tslu_final_kernel<T, NC_>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, P, ts, ipiv, srcmap, \
+                       moves, info)
+    if (nc <= 1)
+        LMX_TSLU(1);
+    else if (nc <= 2)
+        LMX_TSLU(2);
+    else
+        LMX_TSLU(4);
+#undef LMX_TSLU
+    LMX_LAUNCH_CHECK();
+}
+
 template <class T>
 void gather_rows(hipStream_t s, T *X, int64_t ldx, int c0, int c1, int c2, int c3, const int32_t *moves,
-                 const int32_t *srcmap) {
-    int ncols = (c1 - c0) + (c3 - c2);
-    if (ncols <= 0) return;
+                 const int32_t *srcmap, int32_t *perm) {
+    const int ncols = (c1 - c0) + (c3 - c2);
+    if (ncols <= 0 && !perm) return;
     TimedRegion tr("laswp", s);
-    hipLaunchKernelGGL(row_gather_kernel<T>, dim3((unsigned)cdiv(ncols, 64)), dim3(256), 0, s, X, ldx, c0, c1,
-                       c2, c3, moves, srcmap);
+    hipLaunchKernelGGL(row_gather_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(ncols, 64))), dim3(256), 0, s,
+                       X, ldx, c0, c1, c2, c3, moves, srcmap, perm);
     LMX_LAUNCH_CHECK();
 }
 
@@ -520,18 +814,22 @@ void clear_info(hipStream_t s, int32_t *d_info) {
 // getrf with optional permutation-vector output (d_perm may be null).
 template <class T>
 void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
-                int32_t *d_info) {
+                int32_t *d_info, bool tournament) {
     if (n < 0 || lda < std::max<int64_t>(1, n)) fail_arg("getrf: bad dimension or stride");
     if (n > INT32_MAX / 2) fail_arg("getrf: n too large");
+    if (tournament && sizeof(T) != 4) fail_arg("getrf: tournament pivoting is for the FP32 factor");
     clear_info(s, d_info);
     if (n == 0) return;
     TimedRegion trall("getrf", s, 2.0 / 3.0 * (double)n * (double)n * (double)n);
     const int N = (int)n;
     size_t sync_bytes = (sizeof(PanelSync<T>) + 255) & ~size_t(255);
-    Scratch scr(sync_bytes + sizeof(int32_t) * ((size_t)N + 2 * KB + 8) + 256, s);
+    const size_t map_bytes = (sizeof(int32_t) * ((size_t)N + 2 * KB + 8) + 255) & ~size_t(255);
+    Scratch scr(sync_bytes + map_bytes + (tournament ? sizeof(TsluScratch<T>) : 0) + 256, s);
     auto *sy = scr.as<PanelSync<T>>();
     int32_t *srcmap = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scr.p) + sync_bytes);
     int32_t *moves = srcmap + N;
+    auto *ts = tournament ? reinterpret_cast<TsluScratch<T> *>(reinterpret_cast<char *>(scr.p) + sync_bytes + map_bytes)
+                          : nullptr;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)cdiv(sync_clear_words<T>(), 256)), dim3(256), 0, s,
                        (unsigned *)sy, sync_clear_words<T>());
     LMX_LAUNCH_CHECK();
@@ -546,12 +844,16 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             const int sb = std::min(KB, k0 + nbw - s0);
             const int m = N - s0;
             T *P0 = A + (int64_t)s0 * lda + s0;
-            launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
-            gather_rows<T>(s, A + (int64_t)s0 * lda, lda, 0, s0, s0 + sb, N, moves, srcmap);
-            if (d_perm) {
-                hipLaunchKernelGGL(perm_gather_kernel, dim3(1), dim3(64), 0, s, d_perm + s0, moves, srcmap);
-                LMX_LAUNCH_CHECK();
+            if constexpr (sizeof(T) == 4) {
+                if (tournament)
+                    launch_tslu<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, ts);
+                else
+                    launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
+            } else {
+                launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
             }
+            gather_rows<T>(s, A + (int64_t)s0 * lda, lda, 0, s0, s0 + sb, N, moves, srcmap,
+                           d_perm ? d_perm + s0 : nullptr);
             const int rest = k0 + nbw - (s0 + sb);
             if (rest > 0) {
                 T *B12 = A + (int64_t)s0 * lda + s0 + sb;
@@ -615,8 +917,8 @@ void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda, con
 
 template void getrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *);
 template void getrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *);
-template void getrf_impl<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *, int32_t *);
-template void getrf_impl<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *, int32_t *);
+template void getrf_impl<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *, int32_t *, bool);
+template void getrf_impl<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *, int32_t *, bool);
 template void getrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const int32_t *, double *,
                             int64_t);
 template void getrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const int32_t *, float *,

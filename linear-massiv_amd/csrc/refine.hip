@@ -8,7 +8,7 @@
 //           eta = max_c ||r_c||_inf / (||A||_inf ||x_c||_inf + ||b_c||_inf)
 // until every column meets LAPACK dsgesv's test
 //   ||r_c||_inf <= sqrt(n) eps ||A||_inf ||x_c||_inf,
-// or eta stops halving, or 30 steps.  A solve that does not converge is
+// or eta stops shrinking (less than 10% per step), or 30 steps.  A solve that does not converge is
 // redone with the FP64 factorisation (fell_back = 1).  Only eta (one double)
 // crosses to the host per step.
 #include <cmath>
@@ -291,7 +291,7 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
                 out.converged = true;
                 break;
             }
-            if (!(h[0] < 0.5 * prev)) break;  // stagnation (or NaN)
+            if (!(h[0] < 0.9 * prev)) break;  // stagnation (or NaN)
             prev = h[0];
         }
     }

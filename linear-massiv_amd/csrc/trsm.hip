@@ -25,123 +25,103 @@ namespace {
 
 constexpr int TB = 64;
 
-// Element (r, c) of M = op(T) (MT = transpose), restricted to M's triangle.
-template <class T, bool MT, bool MLOWER>
-__device__ inline T m_elem(const T *A, int64_t lda, int64_t r, int64_t c) {
-    bool in = MLOWER ? (c <= r) : (c >= r);
-    if (!in) return T(0);
-    return MT ? A[c * lda + r] : A[r * lda + c];
+// A diagonal block of M = op(T) in LDS as a LOWER triangle of order S:
+// M restricted to its triangle and padded with the identity beyond nb (nb
+// <= 0: the identity); an upper M is staged reversed, Ms[r][c] =
+// M[S-1-r][S-1-c] (J M J with J the reversal is lower), so one forward
+// solve serves both.  Thread t reads column t of each stored row i --
+// coalesced -- which is M[i][t], or M[t][i] for a transposed operand.
+// Rows are 16-byte aligned so the solve reads four coefficients per access.
+constexpr int ldm_of(int bytes) { return TB + 16 / bytes; }
+template <class T>
+__device__ inline void stage_block(T (*Ms)[ldm_of(sizeof(T))], const T *A, int64_t lda, int nb, int t, bool mt,
+                                   bool mlower, bool unit, int S) {
+#pragma unroll 16
+    for (int i = 0; i < TB; ++i) {
+        const int r = mt ? t : i, c = mt ? i : t;
+        if (r >= S || c >= S) continue;
+        T v;
+        if (r < nb && c < nb) {
+            const bool in = mlower ? c <= r : c >= r;
+            v = (in && !(unit && r == c)) ? A[(int64_t)i * lda + t] : T(0);
+            if (unit && r == c) v = T(1);
+        } else {
+            v = r == c ? T(1) : T(0);
+        }
+        if (mlower)
+            Ms[r][c] = v;
+        else
+            Ms[S - 1 - r][S - 1 - c] = v;
+    }
+}
+
+// L x = b in place on the registers x[0..S), L lower in LDS: row form
+// x_r = (b_r - sum_{k<r} l_rk x_k) / l_rr, k ascending (for a reversed
+// upper block: the reference back substitution with its sum taken from
+// the far end).  A unit diagonal is staged as 1; dividing by 1 is exact.
+template <class T, int S>
+__device__ __forceinline__ void solve_regs(const T (*Ms)[ldm_of(sizeof(T))], T (&x)[TB]) {
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+        T s = x[r];
+#pragma unroll
+        for (int k = 0; k < r; ++k) s = fma(-Ms[r][k], x[k], s);
+        x[r] = s / Ms[r][r];
+    }
+}
+
+// M x = b for one diagonal block (order nb <= S <= 64) and the lane's
+// right-hand-side vector, element r at p[r rs]: the vector is held in
+// registers (reversed for an upper M), the block in LDS.  The results go
+// back through the lane's own LDS column: a per-row guard on stores from
+// registers costs the compiler its register allocation.
+template <class T, int S>
+__device__ __forceinline__ void leaf_body(int nb, bool live, const T *__restrict__ A, int64_t lda, T *p, int64_t rs,
+                                          bool mt, bool mlower, bool unit) {
+    __shared__ __attribute__((aligned(16))) T Ms[TB][ldm_of(sizeof(T))];
+    __shared__ T Xs[TB][TB + 1];
+    const int t = threadIdx.x;
+    stage_block<T>(Ms, A, lda, nb, t, mt, mlower, unit, S);
+    T x[TB];
+#pragma unroll
+    for (int i = 0; i < TB; ++i) {
+        const int r = mlower ? i : S - 1 - i;  // register i holds element r
+        x[i] = T(0);
+        if (i < S && live && r < nb) x[i] = p[r * rs];
+    }
+    __syncthreads();
+    solve_regs<T, S>(Ms, x);
+#pragma unroll
+    for (int i = 0; i < S; ++i) Xs[i][t] = x[i];
+    if (!live) return;
+    for (int r = 0; r < nb; ++r) p[r * rs] = Xs[mlower ? r : S - 1 - r][t];
+}
+
+// M x = b for the diagonal block (order nb <= S) with one right-hand-side
+// vector per thread.  right = 0: vectors are columns of B; right = 1: rows.
+template <class T, int S>
+__global__ __launch_bounds__(64) void trsm_leaf_kernel(int nb, int nvec, const T *__restrict__ A, int64_t lda,
+                                                       T *__restrict__ B, int64_t ldb, int mt, int mlower, int unit,
+                                                       int right) {
+    const int v = blockIdx.x * 64 + threadIdx.x;
+    T *p = right ? B + (int64_t)v * ldb : B + v;
+    leaf_body<T, S>(nb, v < nvec, A, lda, p, right ? 1 : ldb, mt != 0, mlower != 0, unit != 0);
 }
 
 // W_b = inverse of diagonal block b of M (order min(64, n - 64 b), padded
 // to a 64 x 64 identity; the identity when the block starts at or beyond
-// n).  W holds dense wld x wld blocks (row stride wld), and the 64-blocks
-// lie along their diagonals: block b sits in wld-block b / (wld / 64) at
-// diagonal offset 64 (b % (wld / 64)).
-template <class T, bool MT, bool MLOWER, bool UNIT>
-__global__ __launch_bounds__(256) void trtri_blocks_kernel(int n, const T *__restrict__ A, int64_t lda,
-                                                           T *__restrict__ W, int wld) {
-    __shared__ T Ms[TB][TB + 1];
-    __shared__ T Xs[TB][TB + 1];
-    __shared__ T red[4][TB];
-    const int t = threadIdx.x, b = blockIdx.x;
-    const int r0 = b * TB, nb = min(TB, n - r0);
-    const T *Ab = A + (int64_t)r0 * lda + r0;
-#pragma unroll
-    for (int e = t; e < TB * TB; e += 256) {
-        int r = e >> 6, c = e & 63;
-        T v = T(0);
-        if (r < nb && c < nb) {
-            v = m_elem<T, MT, MLOWER>(Ab, lda, r, c);
-            if (r == c && UNIT) v = T(1);
-        } else if (r == c) {
-            v = T(1);
-        }
-        Ms[r][c] = v;
-        Xs[r][c] = T(0);
-    }
-    __syncthreads();
-    const int c = t & 63, q = t >> 6;
-    for (int step = 0; step < TB; ++step) {
-        const int r = MLOWER ? step : TB - 1 - step;
-        T s = T(0);
-        if (MLOWER) {
-            for (int k = c + q; k < r; k += 4) s = fma(Ms[r][k], Xs[k][c], s);
-        } else {
-            for (int k = r + 1 + q; k <= c; k += 4) s = fma(Ms[r][k], Xs[k][c], s);
-        }
-        red[q][c] = s;
-        __syncthreads();
-        if (q == 0) {
-            bool in = MLOWER ? (r >= c) : (r <= c);
-            T v = T(0);
-            if (in) {
-                T tot = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
-                v = ((r == c ? T(1) : T(0)) - tot) / Ms[r][r];
-            }
-            Xs[r][c] = v;
-        }
-        __syncthreads();
-    }
-    const int per = wld / TB, zk = b / per, i = b % per;
-    T *Wb = W + (int64_t)zk * wld * wld + (int64_t)i * TB * wld + i * TB;
-    for (int e = t; e < TB * TB; e += 256) Wb[(int64_t)(e >> 6) * wld + (e & 63)] = Xs[e >> 6][e & 63];
-}
-
-// ---------------------------------------------------------------- leaf kernel
-
-// M x = b for the diagonal block (order nb <= 64) with one right-hand-side
-// vector per thread: the block sits in LDS, each thread's vector in its own
-// LDS column, so after the block is staged no barrier is needed.  Row form:
-// x_r = (b_r - sum_k m_rk x_k) / m_rr (a division, as the reference's
-// substitutions).  LEFT: vectors are columns of B; RIGHT: rows of B.
-template <class T, bool MT, bool MLOWER, bool UNIT, bool RIGHT>
-__global__ __launch_bounds__(64) void trsm_leaf_kernel(int nb, int nvec, const T *__restrict__ A, int64_t lda,
-                                                       T *__restrict__ B, int64_t ldb) {
-    constexpr int PAD = sizeof(T) == 8 ? 1 : 2;
-    __shared__ T Ms[TB][TB + PAD];
-    __shared__ T Xs[TB][64 + PAD];
-    const int t = threadIdx.x;
-    // unrolled so that the global loads of a batch are in flight together
-#pragma unroll 16
-    for (int e = t; e < TB * TB; e += 64) {
-        int r = e >> 6, c = e & 63;
-        T v = T(0);
-        if (r < nb && c < nb) {
-            v = m_elem<T, MT, MLOWER>(A, lda, r, c);
-            if (r == c && UNIT) v = T(1);
-        }
-        Ms[r][c] = v;
-    }
-    const int v = blockIdx.x * 64 + t;
-    const bool live = v < nvec;
-    if (RIGHT) {
-#pragma unroll 16
-        for (int r = 0; r < nb; ++r) Xs[r][t] = live ? B[(int64_t)v * ldb + r] : T(0);
-    } else {
-#pragma unroll 16
-        for (int r = 0; r < nb; ++r) Xs[r][t] = live ? B[(int64_t)r * ldb + v] : T(0);
-    }
-    __syncthreads();
-    if (MLOWER) {
-        for (int r = 0; r < nb; ++r) {
-            T s = Xs[r][t];
-            for (int k = 0; k < r; ++k) s = fma(-Ms[r][k], Xs[k][t], s);
-            Xs[r][t] = UNIT ? s : s / Ms[r][r];
-        }
-    } else {
-        for (int r = nb - 1; r >= 0; --r) {
-            T s = Xs[r][t];
-            for (int k = r + 1; k < nb; ++k) s = fma(-Ms[r][k], Xs[k][t], s);
-            Xs[r][t] = UNIT ? s : s / Ms[r][r];
-        }
-    }
-    if (!live) return;
-    if (RIGHT) {
-        for (int r = 0; r < nb; ++r) B[(int64_t)v * ldb + r] = Xs[r][t];
-    } else {
-        for (int r = 0; r < nb; ++r) B[(int64_t)r * ldb + v] = Xs[r][t];
-    }
+// n), solved in place over the identity the diagonal copy left there: lane
+// t solves M x = e_t, giving column t.  W holds dense wld x wld blocks (row
+// stride wld), and the 64-blocks lie along their diagonals: block b sits
+// in wld-block b / (wld / 64) at diagonal offset 64 (b % (wld / 64)).
+template <class T>
+__global__ __launch_bounds__(64) void trtri_blocks_kernel(int n, const T *__restrict__ A, int64_t lda,
+                                                          T *__restrict__ W, int wld, int mt, int mlower, int unit) {
+    const int b = blockIdx.x, r0 = b * TB;
+    const int per = wld / TB, zk = b / per, i0 = b % per;
+    T *Wb = W + (int64_t)zk * wld * wld + (int64_t)i0 * TB * wld + i0 * TB;
+    leaf_body<T, TB>(min(TB, n - r0), true, A + (int64_t)r0 * lda + r0, lda, Wb + threadIdx.x, wld, mt != 0,
+                     mlower != 0, unit != 0);
 }
 
 // ---------------------------------------------------------------- block inverses
@@ -151,7 +131,9 @@ __global__ __launch_bounds__(64) void trsm_leaf_kernel(int nb, int nvec, const T
 constexpr int IBW = 1024;
 
 // W (nbw dense blocks of bw x bw, row stride bw) = the diagonal bw-blocks of
-// M = op(T) restricted to M's strict triangle, zero elsewhere and beyond n.
+// M = op(T) restricted to M's strict triangle, zero elsewhere and beyond n,
+// with the identity in the diagonal 64 x 64 tiles (the 64-block inverses
+// are solved in place over it).
 // One workgroup per 64 x 64 tile (blockIdx.y = diagonal block); a
 // transposed operand goes through LDS so both sides stay coalesced.
 template <class T, bool MT, bool MLOWER>
@@ -181,8 +163,8 @@ __global__ __launch_bounds__(256) void tri_diag_copy_kernel(int n, int bw, const
     for (int e = t; e < TB * TB; e += 256) {
         const int r = e >> 6, c = e & 63;
         const int64_t gr = r0 + r, gc = c0 + c;
-        const bool in = live && (MLOWER ? gc < gr : gc > gr);
-        Wt[(int64_t)r * bw + c] = in ? tile[r][c] : T(0);
+        const bool in = live && (MLOWER ? gc < gr : gc > gr) && ti != tj;
+        Wt[(int64_t)r * bw + c] = in ? tile[r][c] : (ti == tj && r == c ? T(1) : T(0));
     }
 }
 
@@ -247,22 +229,9 @@ __global__ void copy_rows_kernel(const T *__restrict__ in, int64_t ldi, T *__res
 template <class T>
 void trtri_blocks(hipStream_t s, bool mt, bool mlower, bool unit, int n, int nblocks, const T *A, int64_t lda, T *W,
                   int wld) {
-    dim3 g((unsigned)nblocks), blk(256);
-#define LMX_TRTRI(MT_, ML_, U_)                                                                      \
-    if (mt == MT_ && mlower == ML_ && unit == U_) {                                                 \
-        hipLaunchKernelGGL((trtri_blocks_kernel<T, MT_, ML_, U_>), g, blk, 0, s, n, A, lda, W, wld); \
-        LMX_LAUNCH_CHECK();                                                                          \
-        return;                                                                                      \
-    }
-    LMX_TRTRI(false, false, false)
-    LMX_TRTRI(false, false, true)
-    LMX_TRTRI(false, true, false)
-    LMX_TRTRI(false, true, true)
-    LMX_TRTRI(true, false, false)
-    LMX_TRTRI(true, false, true)
-    LMX_TRTRI(true, true, false)
-    LMX_TRTRI(true, true, true)
-#undef LMX_TRTRI
+    hipLaunchKernelGGL(trtri_blocks_kernel<T>, dim3((unsigned)nblocks), dim3(64), 0, s, n, A, lda, W, wld, mt ? 1 : 0,
+                       mlower ? 1 : 0, unit ? 1 : 0);
+    LMX_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -310,27 +279,17 @@ void apply_block(hipStream_t s, int64_t rows, int64_t k, int64_t nrhs, T alpha, 
     LMX_LAUNCH_CHECK();
 }
 
-// Leaf solve of one diagonal block (order bs) for nvec vectors.
+// Leaf solve of one diagonal block (order bs <= 64) for nvec vectors.
 template <class T, bool RIGHT>
 void leaf(hipStream_t s, bool mt, bool mlower, bool unit, int bs, int64_t nvec, const T *A, int64_t lda, T *B,
           int64_t ldb) {
     dim3 g((unsigned)cdiv(nvec, 64)), blk(64);
-#define LMX_LEAF(MT_, ML_, U_)                                                                                  \
-    if (mt == MT_ && mlower == ML_ && unit == U_) {                                                            \
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, MT_, ML_, U_, RIGHT>), g, blk, 0, s, bs, (int)nvec, A, lda, B, \
-                           ldb);                                                                               \
-        LMX_LAUNCH_CHECK();                                                                                     \
-        return;                                                                                                 \
-    }
-    LMX_LEAF(false, false, false)
-    LMX_LEAF(false, false, true)
-    LMX_LEAF(false, true, false)
-    LMX_LEAF(false, true, true)
-    LMX_LEAF(true, false, false)
-    LMX_LEAF(true, false, true)
-    LMX_LEAF(true, true, false)
-    LMX_LEAF(true, true, true)
-#undef LMX_LEAF
+    const int imt = mt ? 1 : 0, iml = mlower ? 1 : 0, iu = unit ? 1 : 0, ir = RIGHT ? 1 : 0;
+    if (bs <= 32)
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, 32>), g, blk, 0, s, bs, (int)nvec, A, lda, B, ldb, imt, iml, iu, ir);
+    else
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, 64>), g, blk, 0, s, bs, (int)nvec, A, lda, B, ldb, imt, iml, iu, ir);
+    LMX_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -407,60 +366,57 @@ void tri_inverse_solve(hipStream_t s, bool lower, bool trans, int64_t n, int64_t
     LMX_LAUNCH_CHECK();
 }
 
-// op(T) X = B, B n x nrhs.
+// op(T) X = B, B n x nrhs: one leaf launch for a single 64-block, the
+// block-inverse path otherwise.
 template <class T>
 void trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int64_t n, int64_t nrhs, const T *A, int64_t lda,
                T *B, int64_t ldb) {
     if (n <= 0 || nrhs <= 0) return;
     if (n > INT32_MAX / 2 || nrhs > INT32_MAX / 2) fail_arg("trsm: dimension too large");
     TimedRegion tr("trsm", s, (double)n * (double)n * (double)nrhs);
-    const bool mt = trans, mlower = lower != trans;
-    const int64_t nblk = cdiv(n, TB);
-    if (nrhs <= 64 && n > 4 * TB) {
-        Scratch sw((size_t)tri_inverse_elems(n) * sizeof(T), s);
-        tri_inverse_build<T>(s, lower, trans, unit, n, A, lda, sw.as<T>());
-        tri_inverse_solve<T>(s, lower, trans, n, nrhs, A, lda, sw.as<T>(), B, ldb);
+    if (n <= TB) {
+        leaf<T, false>(s, trans, lower != trans, unit, (int)n, nrhs, A, lda, B, ldb);
         return;
     }
-    // M[r0.., c0..] as a GEMM operand: the T block, transposed when trans.
-    auto mblock = [&](int64_t r0, int64_t c0) -> const T * { return trans ? A + c0 * lda + r0 : A + r0 * lda + c0; };
-    for (int64_t bi = 0; bi < nblk; ++bi) {
-        int64_t b = mlower ? bi : nblk - 1 - bi;
-        int64_t b0 = b * TB, bs = std::min<int64_t>(TB, n - b0);
-        leaf<T, false>(s, mt, mlower, unit, (int)bs, nrhs, A + b0 * lda + b0, lda, B + b0 * ldb, ldb);
-        if (mlower && b0 + bs < n) {
-            int64_t r0 = b0 + bs, rn = n - r0;
-            gemm(s, trans, false, rn, nrhs, bs, T(-1), mblock(r0, b0), lda, B + b0 * ldb, ldb, T(1), B + r0 * ldb,
-                 ldb);
-        } else if (!mlower && b0 > 0) {
-            gemm(s, trans, false, b0, nrhs, bs, T(-1), mblock(0, b0), lda, B + b0 * ldb, ldb, T(1), B, ldb);
-        }
-    }
+    Scratch sw((size_t)tri_inverse_elems(n) * sizeof(T), s);
+    tri_inverse_build<T>(s, lower, trans, unit, n, A, lda, sw.as<T>());
+    tri_inverse_solve<T>(s, lower, trans, n, nrhs, A, lda, sw.as<T>(), B, ldb);
 }
 
-// X op(T) = B with B nrows x n (row-vector equations M x = b, M = op(T)^T).
+// X op(T) = B with B nrows x n.  A single 64-block is one leaf launch
+// (row-vector equations M x = b, M = op(T)^T).  Otherwise, with R = op(T)
+// and W_j the inverses of R's diagonal blocks, in dependency order (last
+// block first for a lower R): X_j = B_j W_j, then the unsolved columns take
+// B_i -= X_j R_ji -- MFMA GEMMs over all rows at once.
 template <class T>
 void trsm_right(hipStream_t s, bool lower, bool trans, bool unit, int64_t nrows, int64_t n, const T *A, int64_t lda,
                 T *B, int64_t ldb) {
     if (n <= 0 || nrows <= 0) return;
     if (n > INT32_MAX / 2 || nrows > INT32_MAX / 2) fail_arg("trsm: dimension too large");
     TimedRegion tr("trsm", s, (double)n * (double)n * (double)nrows);
-    const bool mt = !trans, mlower = lower != mt;
-    const int64_t nblk = cdiv(n, TB);
-    for (int64_t bi = 0; bi < nblk; ++bi) {
-        int64_t b = mlower ? bi : nblk - 1 - bi;
-        int64_t b0 = b * TB, bs = std::min<int64_t>(TB, n - b0);
-        leaf<T, true>(s, mt, mlower, unit, (int)bs, nrows, A + b0 * lda + b0, lda, B + b0, ldb);
-        // X[:, later] -= X[:, blk] M[later, blk]^T ; M[r][c] = mt ? T[c][r] : T[r][c]
-        if (mlower && b0 + bs < n) {
-            int64_t c0 = b0 + bs, cn = n - c0;
-            const T *op = mt ? A + b0 * lda + c0 : A + c0 * lda + b0;
-            gemm(s, false, !mt, nrows, cn, bs, T(-1), B + b0, ldb, op, lda, T(1), B + c0, ldb);
-        } else if (!mlower && b0 > 0) {
-            const T *op = mt ? A + b0 * lda : A + b0;
-            gemm(s, false, !mt, nrows, b0, bs, T(-1), B + b0, ldb, op, lda, T(1), B, ldb);
-        }
+    if (n <= TB) {
+        const bool mt = !trans;
+        leaf<T, true>(s, mt, lower != mt, unit, (int)n, nrows, A, lda, B, ldb);
+        return;
     }
+    const bool rlower = lower != trans;
+    const int bw = tri_block(n), nbw = (int)cdiv(n, bw);
+    Scratch sw((size_t)tri_inverse_elems(n) * sizeof(T), s), sX((size_t)nrows * n * sizeof(T), s);
+    T *W = sw.as<T>(), *X = sX.as<T>();
+    tri_inverse_build<T>(s, lower, trans, unit, n, A, lda, W);
+    // R[r0.., c0..] as the right GEMM operand (transposed when trans)
+    auto rblock = [&](int64_t r0, int64_t c0) -> const T * { return trans ? A + c0 * lda + r0 : A + r0 * lda + c0; };
+    for (int bi = 0; bi < nbw; ++bi) {
+        const int b = rlower ? nbw - 1 - bi : bi;
+        const int64_t j0 = (int64_t)b * bw, nj = std::min<int64_t>(bw, n - j0);
+        gemm(s, false, false, nrows, nj, nj, T(1), B + j0, ldb, W + (int64_t)b * bw * bw, bw, T(0), X + j0, n);
+        const int64_t i0 = rlower ? 0 : j0 + nj, ni = rlower ? j0 : n - (j0 + nj);
+        if (ni > 0)
+            gemm(s, false, trans, nrows, ni, nj, T(-1), X + j0, n, rblock(j0, i0), lda, T(1), B + i0, ldb);
+    }
+    hipLaunchKernelGGL(copy_rows_kernel<T>, dim3((unsigned)cdiv(nrows * n, 256)), dim3(256), 0, s, X, n, B, ldb,
+                       nrows, (int)n);
+    LMX_LAUNCH_CHECK();
 }
 
 template void trsm_left<double>(hipStream_t, bool, bool, bool, int64_t, int64_t, const double *, int64_t, double *,
