@@ -43,7 +43,7 @@ constexpr int PT = 512;       // threads per panel workgroup (8 waves)
 constexpr int PW = PT / 64;   // waves per panel workgroup
 constexpr int PANEL_ROWS = PT * RPT;
 constexpr int MAXP = 64;      // max cooperative workgroups (panel height <= 32768)
-constexpr int NB_OUTER = 128; // outer panel width (k of the trailing GEMM)
+constexpr int NB_OUTER = 256; // outer panel width (k of the trailing GEMM)
 constexpr long SPIN_LIMIT = 1L << 22;
 
 // Cross-workgroup records of the panel kernel.  Every word is 64 bits: a
