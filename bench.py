@@ -31,8 +31,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "linear-massiv_amd"))
 
 EPS = 2.220446049250313e-16
-PEAK = {"gemm": 78.6, "sgemm": 157.3, "syrk": 78.6}  # TFLOP/s, MI355X dense MFMA
-KERNEL_FAMILIES = ["gemm", "sgemm", "syrk", "getrf_panel", "trsm", "laswp", "potrf_diag"]
+PEAK = {"gemm": 78.6, "sgemm": 157.3, "syrk": 78.6, "ssyrk": 157.3}  # TFLOP/s, MI355X dense MFMA
+KERNEL_FAMILIES = ["gemm", "sgemm", "gemm64", "sgemm64", "syrk", "ssyrk", "getrf_panel", "trsm", "laswp",
+                   "potrf_diag"]
+# HBM bytes per launch of a workload's dominant kernel, from the rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (workload, n) -> file
+TRAFFIC_PROFILES = {("lu_mixed", 16384): "sgemm_lu_mixed_n16384_traffic.md"}
+
+
+def profiled_traffic(workload, n):
+    name = TRAFFIC_PROFILES.get((workload, n))
+    if not name:
+        return None
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def parse():
@@ -541,16 +558,19 @@ def main():
     lmx.kernel_timing_reset()
 
     # dominant MFMA kernel family and its roofline
-    mf = [f for f in ("gemm", "sgemm", "syrk") if f in fam]
+    mf = [f for f in ("gemm", "sgemm", "syrk", "ssyrk") if f in fam]
     dom = max(mf, key=lambda f: fam[f]["ms_per_step"]) if mf else None
     roofline = None
     if dom:
         cnt, ms, work = fam[dom]["launches"], fam[dom]["ms_per_step"], fam[dom]["work_per_step"]
         achieved = work / (ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "kernel": {"gemm": "gemm_mfma_kernel<double>", "sgemm": "gemm_mfma_kernel<float>",
-                                                "syrk": "gemm_mfma_kernel<double> (lower tiles)"}[dom],
+        roofline = {"bound": "mfma",
+                    "kernel": {"gemm": "gemm_mfma_kernel<double, 128, 128, 16>",
+                               "sgemm": "gemm_mfma_kernel<float, 128, 128, 16>",
+                               "syrk": "gemm_mfma_kernel<double> (lower tiles)",
+                               "ssyrk": "gemm_mfma_kernel<float> (lower tiles)"}[dom],
                     "achieved": round(achieved, 3), "peak": PEAK[dom], "unit": "TFLOP/s",
-                    "frac": round(achieved / PEAK[dom], 4), "traffic": None,
+                    "frac": round(achieved / PEAK[dom], 4), "traffic": profiled_traffic(wl, n),
                     "launches_per_step": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2)}
 
     result = {"metric": "FP64-accurate solves/sec & TFLOP/s vs plain FP64 GPU; p50/p99 solve latency",

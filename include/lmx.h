@@ -322,9 +322,12 @@ int lmx_server_destroy(lmx_server *server);
 /* When enabled, the library brackets every launch of the named kernel
  * family with HIP events on the launch stream and accumulates the elapsed
  * time and the algorithmic work (flops) of each launch.  Kernel families:
- * "gemm" (FP64 MFMA GEMM), "sgemm" (FP32), "syrk", "getrf_panel", "trsm",
- * "laswp", "potrf_diag"; whole-call families: "getrf", "potrf", "getrs",
- * "refine".  No reference counterpart. */
+ * "gemm" / "sgemm" (FP64 / FP32 MFMA GEMM, 128 x 128 tiles), "gemm64" /
+ * "sgemm64" (the same with 64 x 64 tiles, and the batched GEMMs), "syrk" /
+ * "ssyrk" (FP64 / FP32),
+ * "getrf_panel", "trsm", "laswp", "potrf_diag", "batch"; whole-call
+ * families: "getrf", "potrf", "getrs", "refine".  No reference
+ * counterpart. */
 int lmx_kernel_timing(int enable);
 int lmx_kernel_timing_reset(void);
 /* Count of timed launches, their total milliseconds and total work for one

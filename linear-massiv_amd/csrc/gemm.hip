@@ -258,16 +258,20 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
     if (m == 0 || n == 0) return;
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
-    TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, 2.0 * (double)m * (double)n * (double)k);
     // Large outputs: 128 x 128 tiles.  Smaller ones: 64 x 64 tiles so the
-    // launch still spreads over the 256 CUs.
+    // launch still spreads over the 256 CUs.  Each tile size is its own
+    // timing family (one kernel each).
+    const double flops = 2.0 * (double)m * (double)n * (double)k;
     int64_t big_tiles = cdiv(m, 128) * cdiv(n, 128);
-    if (big_tiles >= 512)
+    if (big_tiles >= 512) {
+        TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, flops);
         launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
                                      C, ldc);
-    else
+    } else {
+        TimedRegion tr(sizeof(T) == 8 ? "gemm64" : "sgemm64", s, flops);
         launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
                                    C, ldc);
+    }
 }
 
 template <class T>
@@ -276,7 +280,7 @@ void gemm_batched_impl(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, 
                        int64_t b1, int64_t c1, int64_t a2, int64_t b2, int64_t c2) {
     if (m <= 0 || n <= 0 || nb1 <= 0 || nb2 <= 0) return;
     if ((int64_t)nb1 * nb2 > 65535) fail_arg("gemm_batched: too many batch entries");
-    TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, 2.0 * (double)m * (double)n * (double)k * nb1 * nb2);
+    TimedRegion tr(sizeof(T) == 8 ? "gemm64" : "sgemm64", s, 2.0 * (double)m * (double)n * (double)k * nb1 * nb2);
     BatchStrides bs{nb1, a1, b1, c1, a2, b2, c2};
     launch_tile<T, 64, 64, 16>(s, false, false, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc, 0,
                                nb1 * nb2, bs);
@@ -287,7 +291,7 @@ void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t
                int64_t ldc) {
     if (n <= 0) return;
     if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
-    TimedRegion tr("syrk", s, (double)n * (double)(n + 1) * (double)k);
+    TimedRegion tr(sizeof(T) == 8 ? "syrk" : "ssyrk", s, (double)n * (double)(n + 1) * (double)k);
     int64_t big_tiles = cdiv(n, 128) * (cdiv(n, 128) + 1) / 2;
     if (big_tiles >= 512)
         launch_tile<T, 128, 128, 16>(s, false, true, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,

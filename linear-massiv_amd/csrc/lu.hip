@@ -21,10 +21,10 @@
 // the matrix (and the permutation vector) is permuted with one parallel
 // gather instead of a sequential swap loop.
 //
-// Driver (getrf_impl): outer panels of NB = 128 columns, each factored as
-// four 32-column sub-panels (panel kernel, gather, TRSM + GEMM update
-// inside the outer panel), then U12 = L11^{-1} A12 (blocked TRSM) and the
-// trailing update A22 -= L21 U12 on the MFMA GEMM.
+// Driver (getrf_impl): outer panels of NB = 256 columns, each factored as
+// eight 32-column sub-panels (panel kernel, gather, TRSM + GEMM update
+// inside the outer panel), then U12 = L11^{-1} A12 (TRSM through the
+// inverse of L11) and the trailing update A22 -= L21 U12 on the MFMA GEMM.
 #include <climits>
 #include <vector>
 

@@ -13,7 +13,7 @@ Context *g_ctx[kMaxDevices] = {};
 std::mutex g_ctx_mu;
 
 const char *kFamilies[] = {"gemm", "sgemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
-                           "syrk", "getrs", "refine", "getrf", "potrf", "batch"};
+                           "syrk", "getrs", "refine", "getrf", "potrf", "batch", "gemm64", "sgemm64", "ssyrk"};
 constexpr int kNumFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
 struct EventPair {
     hipEvent_t a, b;
