@@ -102,7 +102,7 @@ struct PanelRows {
 // LDS of the panel kernel (double-buffered by column parity).
 template <class T>
 struct PanelLds {
-    T slot_data[2][PW][KB];        // each wave's best row
+    alignas(16) T slot_data[2][PW][KB];  // each wave's best row
     T slot_val[2][PW];
     int slot_row[2][PW];
     int slot_tag[2][PW];
@@ -111,6 +111,30 @@ struct PanelLds {
     int pivs[KB];
     alignas(8) unsigned pay[2][PanelRec<T>::TOTAL];  // polled record payloads (several workgroups)
 };
+
+// The workgroup's best of the PW wave slots: largest value, lowest id on
+// ties.  All slots are loaded before any comparison: compared as they are
+// read, every step waits for its own LDS round trip.
+template <class T>
+__device__ __forceinline__ void wg_select(const T *vals, const int *ids, T &best, int &bid, int &bw) {
+    T v[PW];
+    int id[PW];
+#pragma unroll
+    for (int x = 0; x < PW; ++x) {
+        v[x] = vals[x];
+        id[x] = ids[x];
+    }
+    best = v[0];
+    bid = id[0];
+    bw = 0;
+#pragma unroll
+    for (int x = 1; x < PW; ++x) {
+        const bool take = v[x] > best || (v[x] == best && id[x] < bid);
+        best = take ? v[x] : best;
+        bid = take ? id[x] : bid;
+        bw = take ? x : bw;
+    }
+}
 
 // One column (j = j0 + U) of the panel: pivot search, interchange, scaling
 // and rank-1 update of the rest of the panel.  Returns the pivot row.
@@ -157,16 +181,9 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     }
     __syncthreads();
     // 2. this workgroup's candidate
-    T best = L.slot_val[par][0];
-#pragma unroll
-    for (int x = 1; x < PW; ++x) best = fmax(best, L.slot_val[par][x]);
-    int bw = -1, brow = INT_MAX;
-#pragma unroll
-    for (int x = 0; x < PW; ++x)
-        if (L.slot_val[par][x] == best && L.slot_row[par][x] < brow) {
-            brow = L.slot_row[par][x];
-            bw = x;
-        }
+    T best;
+    int bw, brow;
+    wg_select<T>(L.slot_val[par], L.slot_row[par], best, brow, bw);
     if (!(best >= T(0))) {
         bw = -1;
         brow = INT_MAX;
@@ -410,7 +427,7 @@ struct GeppRows {
 
 template <class T>
 struct GeppLds {
-    T slot_data[2][PW][KB];
+    alignas(16) T slot_data[2][PW][KB];
     T slot_val[2][PW];
     int slot_id[2][PW];
     int piv_id[KB];
@@ -438,32 +455,23 @@ __device__ __forceinline__ void gepp_column(GeppRows<T, NR> &st, GeppLds<T> &L) 
     for (int q = 0; q < NR; ++q)
         if (st.id[q] == wid && st.col[q] < 0) {
 #pragma unroll
-            for (int c = J; c < KB; ++c) L.slot_data[par][wave][c] = st.v[q][c];
+            for (int c = 0; c < KB; ++c) L.slot_data[par][wave][c] = st.v[q][c];  // whole row: vector stores
         }
     if (lane == 0) {
         L.slot_val[par][wave] = wmax;
         L.slot_id[par][wave] = wid;
     }
     __syncthreads();
-    T best = T(-1);
-    int bid = INT_MAX, bw = -1;
-#pragma unroll
-    for (int x = 0; x < PW; ++x) {
-        const T vx = L.slot_val[par][x];
-        const int ix = L.slot_id[par][x];
-        if (vx > best || (vx == best && ix < bid)) {
-            best = vx;
-            bid = ix;
-            bw = x;
-        }
-    }
-    if (bw < 0 || bid == INT_MAX) {  // no row left (uniform)
+    T best;
+    int bid, bw;
+    wg_select<T>(L.slot_val[par], L.slot_id[par], best, bid, bw);
+    if (!(best > T(-1))) {  // no row left (uniform)
         if (t == 0) L.piv_id[J] = -1;
         return;
     }
     T pr[KB];
 #pragma unroll
-    for (int c = J; c < KB; ++c) pr[c] = L.slot_data[par][bw][c];
+    for (int c = 0; c < KB; ++c) pr[c] = L.slot_data[par][bw][c];  // whole row: vector loads
 #pragma unroll
     for (int q = 0; q < NR; ++q)
         if (st.id[q] == bid && st.col[q] < 0) st.col[q] = J;
@@ -496,37 +504,59 @@ __device__ __forceinline__ void gepp_from(GeppRows<T, NR> &st, GeppLds<T> &L, in
 template <class T>
 struct TsluScratch {
     int32_t cand[MAXP * KB];
-    T candv[MAXP * KB * KB];
-    T top[KB * KB];
+    alignas(16) T candv[MAXP * KB * KB];
+    alignas(16) T top[KB * KB];
 };
+
+// A block of PANEL_ROWS panel rows staged in LDS: the global side is read
+// and written row-contiguously (consecutive lanes, consecutive columns),
+// the thread side row per thread (odd row stride: no bank conflicts).
+constexpr int SLD = KB + 1;
+template <class T>
+__device__ inline void stage_in(T (*S)[SLD], const T *__restrict__ A, int64_t lda, int r0, int m, int kb) {
+#pragma unroll 16
+    for (int e = threadIdx.x; e < PANEL_ROWS * KB; e += PT) {
+        const int row = e / KB, c = e % KB, gr = r0 + row;
+        S[row][c] = (gr < m && c < kb) ? A[(int64_t)gr * lda + c] : T(0);
+    }
+}
+template <class T>
+__device__ inline void stage_out(const T (*S)[SLD], T *__restrict__ A, int64_t lda, int r0, int m, int kb) {
+#pragma unroll 16
+    for (int e = threadIdx.x; e < PANEL_ROWS * KB; e += PT) {
+        const int row = e / KB, c = e % KB, gr = r0 + row;
+        if (gr < m && c < kb) A[(int64_t)gr * lda + c] = S[row][c];
+    }
+}
 
 template <class T>
 __global__ __launch_bounds__(PT) void tslu_local_kernel(int m, int kb, const T *__restrict__ A, int64_t lda,
                                                          TsluScratch<T> *__restrict__ ts) {
     const int w = blockIdx.x, t = threadIdx.x;
     __shared__ GeppLds<T> L;
+    __shared__ T S[PANEL_ROWS][SLD];
     GeppRows<T, RPT> st;
+    stage_in<T>(S, A, lda, w * PANEL_ROWS, m, kb);
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         const int r = w * PANEL_ROWS + q * PT + t;
         st.id[q] = r < m ? r : -1;
         st.col[q] = -1;
 #pragma unroll
-        for (int c = 0; c < KB; ++c) st.v[q][c] = (r < m && c < kb) ? A[(int64_t)r * lda + c] : T(0);
+        for (int c = 0; c < KB; ++c) st.v[q][c] = S[q * PT + t][c];
     }
     gepp_from<T, RPT, 0>(st, L, kb);
     __syncthreads();
+    // candidates with their original values (still in S)
     for (int e = t; e < KB * KB; e += PT) {
         const int i = e / KB, c = e % KB;
         const int r = i < kb ? L.piv_id[i] : -1;
         if (c == 0) ts->cand[w * KB + i] = r;
-        ts->candv[(w * KB + i) * KB + c] = (r >= 0 && c < kb) ? A[(int64_t)r * lda + c] : T(0);
+        ts->candv[(w * KB + i) * KB + c] = r >= 0 ? S[r - w * PANEL_ROWS][c] : T(0);
     }
     if (w == 0)
-        for (int e = t; e < KB * KB; e += PT) {
-            const int i = e / KB, c = e % KB;
-            ts->top[e] = (i < m && c < kb) ? A[(int64_t)i * lda + c] : T(0);
-        }
+        for (int e = t; e < KB * KB; e += PT) ts->top[e] = S[e / KB][e % KB];
 }
 
 template <class T, int NC>
@@ -534,13 +564,15 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
                                                          int P, const TsluScratch<T> *__restrict__ ts,
                                                          int32_t *__restrict__ ipiv, int32_t *__restrict__ srcmap,
                                                          int32_t *__restrict__ moves, int32_t *__restrict__ info) {
+    static_assert(sizeof(T) == 4, "the tournament panel is for FP32 factors");
     const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     __shared__ GeppLds<T> L;
-    __shared__ T F[KB][KB + 1];  // the pivot rows, factored: L11 \ U11
+    __shared__ __attribute__((aligned(16))) T F[KB][KB];  // the pivot rows, factored: L11 \ U11
     __shared__ int ent_pos[2 * KB], ent_row[2 * KB];
     __shared__ int s_nent;
+    __shared__ T S[PANEL_ROWS][SLD];
     GeppRows<T, NC> st;
-    for (int e = t; e < KB * (KB + 1); e += PT) (&F[0][0])[e] = T(0);
+    for (int e = t; e < KB * KB; e += PT) (&F[0][0])[e] = T(0);
     const int ncand = P * KB;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
@@ -548,9 +580,18 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
         const int r = ci < ncand ? ts->cand[ci] : -1;
         st.id[q] = r;
         st.col[q] = -1;
+        const float4 *src = reinterpret_cast<const float4 *>(ts->candv + (int64_t)(ci < ncand ? ci : 0) * KB);
 #pragma unroll
-        for (int c = 0; c < KB; ++c) st.v[q][c] = r >= 0 ? ts->candv[ci * KB + c] : T(0);
+        for (int c4 = 0; c4 < KB / 4; ++c4) {
+            const float4 v4 = src[c4];
+            st.v[q][4 * c4] = r >= 0 ? v4.x : T(0);
+            st.v[q][4 * c4 + 1] = r >= 0 ? v4.y : T(0);
+            st.v[q][4 * c4 + 2] = r >= 0 ? v4.z : T(0);
+            st.v[q][4 * c4 + 3] = r >= 0 ? v4.w : T(0);
+        }
     }
+    // own rows, staged while the tournament runs
+    stage_in<T>(S, A, lda, w * PANEL_ROWS, m, kb);
     gepp_from<T, NC, 0>(st, L, kb);
 #pragma unroll
     for (int q = 0; q < NC; ++q)
@@ -560,17 +601,20 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
         }
     __syncthreads();
     // pivot order -> sequential swaps of positions j and cur(p_j); lane e of
-    // wave 0 holds entry e of the (position, row) list of moved positions
+    // wave 0 holds entry e of the (position, row) list of moved positions,
+    // lane j holds pivot j
     if (wave == 0) {
+        const int pj = lane < kb ? L.piv_id[lane] : -1;
         int epos = -1, erow = -1, nent = 0;
         for (int j = 0; j < kb; ++j) {
-            const int r = L.piv_id[j];
+            const int r = __builtin_amdgcn_readlane(pj, j);
             const unsigned long long m1 = __ballot(erow == r);
-            const int cur = m1 ? __shfl(epos, __ffsll((long long)m1) - 1) : r;
+            const int cur = m1 ? __builtin_amdgcn_readlane(epos, __ffsll((long long)m1) - 1) : r;
             const unsigned long long m2 = __ballot(epos == j);
-            const int rj = m2 ? __shfl(erow, __ffsll((long long)m2) - 1) : j;
+            const int l2 = __ffsll((long long)m2) - 1;
+            const int rj = m2 ? __builtin_amdgcn_readlane(erow, l2) : j;
             if (m2) {
-                if (lane == __ffsll((long long)m2) - 1) erow = r;
+                if (lane == l2) erow = r;
             } else {
                 if (lane == nent) {
                     epos = j;
@@ -580,8 +624,9 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
             }
             if (cur != j) {
                 const unsigned long long m3 = __ballot(epos == cur);
+                const int l3 = __ffsll((long long)m3) - 1;
                 if (m3) {
-                    if (lane == __ffsll((long long)m3) - 1) erow = rj;
+                    if (lane == l3) erow = rj;
                 } else {
                     if (lane == nent) {
                         epos = cur;
@@ -616,36 +661,135 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
     const int nent = s_nent;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-        const int pos = w * PANEL_ROWS + q * PT + t;
+        const int row = q * PT + t, pos = w * PANEL_ROWS + row;
         if (pos >= m) continue;
-        T *row = A + (int64_t)pos * lda;
         if (pos < kb) {
 #pragma unroll
-            for (int c = 0; c < KB; ++c)
-                if (c < kb) row[c] = F[pos][c];
+            for (int c = 0; c < KB; ++c) S[row][c] = F[pos][c];
             continue;
         }
         int src = pos;  // a displaced top row, or the row itself
         for (int e = 0; e < nent; ++e)
             if (ent_pos[e] == pos) src = ent_row[e];
-        T x[KB];
         const bool own = src == pos || src < 0 || src >= KB;  // displaced rows come from the first kb
+        T x[KB];
 #pragma unroll
-        for (int c = 0; c < KB; ++c) x[c] = c < kb ? (own ? row[c] : ts->top[src * KB + c]) : T(0);
+        for (int c = 0; c < KB; ++c) x[c] = own ? S[row][c] : ts->top[src * KB + c];
+        // GEPP's updates of a row never chosen, branch-free: l_c = a_c / u_cc,
+        // skipped (no division, no update) when u_cc = 0 as in the panel
 #pragma unroll
         for (int c = 0; c < KB; ++c) {
             const T u = F[c][c];
-            if (c < kb && u != T(0)) {
-                const T l = x[c] / u;
-                x[c] = l;
+            const bool ok = c < kb && u != T(0);
+            const T l = x[c] / (ok ? u : T(1));
+            x[c] = ok ? l : x[c];
+            const T use = ok ? l : T(0);
 #pragma unroll
-                for (int k = c + 1; k < KB; ++k) x[k] = fma(-l, F[c][k], x[k]);
+            for (int k = c + 1; k < KB; ++k) x[k] = fma(-use, F[c][k], x[k]);
+        }
+#pragma unroll
+        for (int c = 0; c < KB; ++c) S[row][c] = x[c];
+    }
+    __syncthreads();
+    stage_out<T>(S, A, lda, w * PANEL_ROWS, m, kb);
+}
+
+// ---------------------------------------------------------------- inner update
+
+// The update inside an outer panel after a 32-column sub-panel, fused into
+// one launch: with L11 the sub-panel's unit-lower 32 x 32 block at P0 and
+// B12 the 32 x rest block to its right, U12 = L11^{-1} B12 (each workgroup
+// solves all of it for itself: lane c solves column c in registers, the
+// coefficients -- the same for every lane -- come through scalar loads of
+// L11, a read-only view of P0), then the workgroup's 64-row tile of A22
+// (rows 32 + 64 b..) takes A22 -= L21 U12.  U12 goes to `ust` (row stride ldst), not into
+// B12: other workgroups may still be reading B12; the driver copies it into
+// place once the outer panel is done (nothing reads it before).
+constexpr int IU_ROWS = 64;
+constexpr int IU_MAXREST = NB_OUTER - KB;
+template <class T>
+__global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, T *__restrict__ P0, int64_t lda,
+                                                           const T *__restrict__ L11, T *__restrict__ ust,
+                                                           int64_t ldst) {
+    constexpr int kb = KB;
+    __shared__ __attribute__((aligned(16))) T L2[IU_ROWS][KB];
+    __shared__ T Us[KB][IU_MAXREST + 1];
+    const int t = threadIdx.x;
+    const int64_t rt = (int64_t)kb + (int64_t)blockIdx.x * IU_ROWS;  // first tile row, panel-relative
+    for (int e = t; e < IU_ROWS * KB; e += 256) {
+        const int i = e / KB, k = e % KB;
+        L2[i][k] = (rt + i < kb + mrows && k < kb) ? P0[(rt + i) * lda + k] : T(0);
+    }
+    T x[KB];
+    const T *B12 = P0 + kb;
+#pragma unroll
+    for (int i = 0; i < KB; ++i) x[i] = (t < rest && i < kb) ? B12[(int64_t)i * lda + t] : T(0);
+    // this thread's A22 elements (column cl + 64 j, rows g*16 + r of the
+    // tile), loaded now so their latency overlaps the solve
+    const int g = t >> 6, cl = t & 63;
+    constexpr int NJ = (IU_MAXREST + 63) / 64;
+    T pre[NJ][16];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int c = cl + 64 * j;
+            const int64_t row = rt + g * 16 + r;
+            pre[j][r] = (c < rest && row < kb + mrows) ? P0[row * lda + kb + c] : T(0);
+        }
+    __syncthreads();
+    // forward substitution with the unit lower L11: x_i -= l_ik x_k, k ascending
+#pragma unroll
+    for (int i = 1; i < KB; ++i) {
+        T sacc = x[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) sacc = fma(-L11[(int64_t)i * lda + k], x[k], sacc);
+        x[i] = sacc;
+    }
+    if (t < rest) {
+#pragma unroll
+        for (int i = 0; i < KB; ++i) Us[i][t] = x[i];
+        if (blockIdx.x == 0)
+            for (int i = 0; i < kb; ++i) ust[(int64_t)i * ldst + t] = x[i];
+    }
+    __syncthreads();
+    // A22 tile -= L21 U12: thread (g, c) takes 16 rows of column c
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int c = cl + 64 * j;
+        if (c >= rest) break;
+        T acc[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = T(0);
+#pragma unroll 2
+        for (int k = 0; k < KB; k += 4) {
+            const T u0 = Us[k][c], u1 = Us[k + 1][c], u2 = Us[k + 2][c], u3 = Us[k + 3][c];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const T *l = &L2[g * 16 + r][k];
+                acc[r] = fma(l[0], u0, acc[r]);
+                acc[r] = fma(l[1], u1, acc[r]);
+                acc[r] = fma(l[2], u2, acc[r]);
+                acc[r] = fma(l[3], u3, acc[r]);
             }
         }
 #pragma unroll
-        for (int c = 0; c < KB; ++c)
-            if (c < kb) row[c] = x[c];
+        for (int r = 0; r < 16; ++r) {
+            const int64_t row = rt + g * 16 + r;
+            if (row < kb + mrows) P0[row * lda + kb + c] = pre[j][r] - acc[r];
+        }
     }
+}
+
+// U12 blocks of an outer panel from the stash into place: element (r, c) of
+// the nbw x nbw diagonal block with c at or beyond the end of r's sub-panel.
+template <class T>
+__global__ void inner_flush_kernel(int nbw, const T *__restrict__ ust, int64_t ldst, T *__restrict__ A11,
+                                   int64_t lda) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nbw * nbw) return;
+    const int r = e / nbw, c = e % nbw;
+    if (c >= (r / KB + 1) * KB) A11[(int64_t)r * lda + c] = ust[(int64_t)r * ldst + c];
 }
 
 // X[d] <- X[src[d]] for every d of the move list, for columns [c0, c1) and
@@ -657,7 +801,8 @@ template <class T>
 __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
                                                          int c2, int c3, const int32_t *__restrict__ moves,
                                                          const int32_t *__restrict__ srcmap,
-                                                         int32_t *__restrict__ perm) {
+                                                         int32_t *__restrict__ perm, int32_t *__restrict__ pairs_out,
+                                                         int row_off) {
     constexpr int MAXM = 2 * KB;
     constexpr int PER = MAXM / 4;  // moves per thread
     __shared__ int s_dst[MAXM], s_src[MAXM], s_cnt;
@@ -668,6 +813,11 @@ __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int6
         s_dst[t] = d;
         s_src[t] = t < cnt ? srcmap[d] : 0;
         if (t == 0) s_cnt = cnt;
+        if (pairs_out != nullptr && blockIdx.x == 0) {  // the move list, rows absolute, for deferred gathers
+            if (t == 0) pairs_out[0] = cnt;
+            pairs_out[1 + t] = row_off + d;
+            pairs_out[1 + MAXM + t] = row_off + s_src[t];
+        }
     }
     __syncthreads();
     const int cnt = s_cnt;
@@ -695,6 +845,47 @@ __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int6
         if (col >= 0 && e < cnt) X[(int64_t)s_dst[e] * ldx + col] = val[i];
     }
     if (pm) perm[s_dst[t]] = pv;
+}
+
+// The move lists of npanels sub-panels (pairs: cnt, dst[2 KB], src[2 KB],
+// rows absolute, PAIR_STRIDE ints each), applied in order to columns
+// [c0, c1) of X: the row interchanges an outer panel deferred for columns
+// that another stream was still using.
+constexpr int PAIR_STRIDE = 4 * KB + 4;
+constexpr int PANELS_PER_BLOCK = NB_OUTER / KB;
+template <class T>
+__global__ __launch_bounds__(256) void row_gather_multi_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
+                                                               const int32_t *__restrict__ pairs, int npanels) {
+    constexpr int MAXM = 2 * KB;
+    constexpr int PER = MAXM / 4;
+    __shared__ int s_dst[MAXM], s_src[MAXM], s_cnt;
+    const int t = threadIdx.x, cl = t & 63, grp = t >> 6;
+    const int cc = c0 + blockIdx.x * 64 + cl;
+    const int col = cc < c1 ? cc : -1;
+    for (int p = 0; p < npanels; ++p) {
+        const int32_t *pp = pairs + (int64_t)p * PAIR_STRIDE;
+        if (t < 64) {
+            const int cnt = min(pp[0], MAXM);
+            s_dst[t] = t < cnt ? pp[1 + t] : 0;
+            s_src[t] = t < cnt ? pp[1 + MAXM + t] : 0;
+            if (t == 0) s_cnt = cnt;
+        }
+        __syncthreads();
+        const int cnt = s_cnt;
+        T val[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = grp * PER + i;
+            val[i] = (col >= 0 && e < cnt) ? X[(int64_t)s_src[e] * ldx + col] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = grp * PER + i;
+            if (col >= 0 && e < cnt) X[(int64_t)s_dst[e] * ldx + col] = val[i];
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void iota_kernel(int32_t *p, int n) {
@@ -795,12 +986,21 @@ void launch_tslu(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32_
 
 template <class T>
 void gather_rows(hipStream_t s, T *X, int64_t ldx, int c0, int c1, int c2, int c3, const int32_t *moves,
-                 const int32_t *srcmap, int32_t *perm) {
+                 const int32_t *srcmap, int32_t *perm, int32_t *pairs_out, int row_off) {
     const int ncols = (c1 - c0) + (c3 - c2);
-    if (ncols <= 0 && !perm) return;
+    if (ncols <= 0 && !perm && !pairs_out) return;
     TimedRegion tr("laswp", s);
     hipLaunchKernelGGL(row_gather_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(ncols, 64))), dim3(256), 0, s,
-                       X, ldx, c0, c1, c2, c3, moves, srcmap, perm);
+                       X, ldx, c0, c1, c2, c3, moves, srcmap, perm, pairs_out, row_off);
+    LMX_LAUNCH_CHECK();
+}
+
+template <class T>
+void gather_rows_deferred(hipStream_t s, T *A, int64_t lda, int c0, int c1, const int32_t *pairs, int npanels) {
+    if (c1 <= c0 || npanels <= 0) return;
+    TimedRegion tr("laswp", s);
+    hipLaunchKernelGGL(row_gather_multi_kernel<T>, dim3((unsigned)cdiv(c1 - c0, 64)), dim3(256), 0, s, A, lda, c0, c1,
+                       pairs, npanels);
     LMX_LAUNCH_CHECK();
 }
 
@@ -824,12 +1024,20 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
     const int N = (int)n;
     size_t sync_bytes = (sizeof(PanelSync<T>) + 255) & ~size_t(255);
     const size_t map_bytes = (sizeof(int32_t) * ((size_t)N + 2 * KB + 8) + 255) & ~size_t(255);
-    Scratch scr(sync_bytes + map_bytes + (tournament ? sizeof(TsluScratch<T>) : 0) + 256, s);
+    const size_t stash_bytes = (sizeof(T) * NB_OUTER * NB_OUTER + 255) & ~size_t(255);
+    const size_t inv_bytes = 2 * stash_bytes;  // L11^-1 of two outer panels (double-buffered)
+    const size_t pair_bytes = (sizeof(int32_t) * 2 * PANELS_PER_BLOCK * PAIR_STRIDE + 255) & ~size_t(255);
+    Scratch scr(sync_bytes + map_bytes + stash_bytes + inv_bytes + pair_bytes +
+                    (tournament ? sizeof(TsluScratch<T>) : 0) + 256,
+                s);
     auto *sy = scr.as<PanelSync<T>>();
     int32_t *srcmap = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(scr.p) + sync_bytes);
     int32_t *moves = srcmap + N;
-    auto *ts = tournament ? reinterpret_cast<TsluScratch<T> *>(reinterpret_cast<char *>(scr.p) + sync_bytes + map_bytes)
-                          : nullptr;
+    char *cur = reinterpret_cast<char *>(scr.p) + sync_bytes + map_bytes;
+    T *stash = reinterpret_cast<T *>(cur);
+    T *Wall = reinterpret_cast<T *>(cur + stash_bytes);
+    int32_t *pairs_all = reinterpret_cast<int32_t *>(cur + stash_bytes + inv_bytes);
+    auto *ts = tournament ? reinterpret_cast<TsluScratch<T> *>(cur + stash_bytes + inv_bytes + pair_bytes) : nullptr;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)cdiv(sync_clear_words<T>(), 256)), dim3(256), 0, s,
                        (unsigned *)sy, sync_clear_words<T>());
     LMX_LAUNCH_CHECK();
@@ -837,10 +1045,37 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
         hipLaunchKernelGGL(iota_kernel, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, s, d_perm, N);
         LMX_LAUNCH_CHECK();
     }
+    // Lookahead: the caller's stream factors outer panel b + 1 while an
+    // auxiliary stream applies panel b's update to the columns beyond
+    // b + 1.  Row interchanges of a panel reach, on the caller's stream, its
+    // own outer panel at once and the trailing columns once the auxiliary
+    // stream is done with them; the columns of earlier panels (L) get them
+    // on the auxiliary stream, after the GEMM that reads them.
+    //   stream s:   panels(b); wait F(b-1); swaps(b) on [k0+nbw, N); W = L11^-1;
+    //               U12 and A22 -= L21 U12 on panel b+1's columns; record E(b)
+    //   stream aux: wait E(b); swaps(b) on [0, k0); U12 and A22 -= L21 U12 on
+    //               the columns beyond panel b+1; record F(b)
+    hipStream_t aux = nullptr;
+    hipEvent_t evE = nullptr, evF = nullptr;
+    LMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    LMX_HIP(hipEventCreateWithFlags(&evE, hipEventDisableTiming));
+    LMX_HIP(hipEventCreateWithFlags(&evF, hipEventDisableTiming));
+    struct Cleanup {
+        hipStream_t a;
+        hipEvent_t e, f;
+        ~Cleanup() {
+            if (e) (void)hipEventDestroy(e);
+            if (f) (void)hipEventDestroy(f);
+            if (a) (void)hipStreamDestroy(a);
+        }
+    } cleanup{aux, evE, evF};
+    bool pending = false;  // evF marks the auxiliary stream's last block
     unsigned base = 0;
-    for (int k0 = 0; k0 < N; k0 += NB_OUTER) {
+    for (int b = 0, k0 = 0; k0 < N; ++b, k0 += NB_OUTER) {
         const int nbw = std::min(NB_OUTER, N - k0);
-        for (int s0 = k0; s0 < k0 + nbw; s0 += KB) {
+        int32_t *pairs = pairs_all + (int64_t)(b & 1) * PANELS_PER_BLOCK * PAIR_STRIDE;
+        int np = 0;
+        for (int s0 = k0; s0 < k0 + nbw; s0 += KB, ++np) {
             const int sb = std::min(KB, k0 + nbw - s0);
             const int m = N - s0;
             T *P0 = A + (int64_t)s0 * lda + s0;
@@ -852,25 +1087,49 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             } else {
                 launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
             }
-            gather_rows<T>(s, A + (int64_t)s0 * lda, lda, 0, s0, s0 + sb, N, moves, srcmap,
-                           d_perm ? d_perm + s0 : nullptr);
+            // this outer panel's columns now, the rest later (move list kept)
+            gather_rows<T>(s, A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, moves, srcmap,
+                           d_perm ? d_perm + s0 : nullptr, pairs + (int64_t)np * PAIR_STRIDE, s0);
             const int rest = k0 + nbw - (s0 + sb);
             if (rest > 0) {
-                T *B12 = A + (int64_t)s0 * lda + s0 + sb;
-                trsm_left<T>(s, true, false, true, sb, rest, P0, lda, B12, lda);
-                gemm(s, false, false, m - sb, rest, sb, T(-1), P0 + (int64_t)sb * lda, lda, B12, lda, T(1),
-                     B12 + (int64_t)sb * lda, lda);
+                // U12 into the stash at (s0 - k0, s0 - k0 + sb) of the outer block
+                T *ust = stash + (int64_t)(s0 - k0) * NB_OUTER + (s0 - k0) + sb;
+                if (sb != KB) fail_arg("getrf: inner update needs a full sub-panel");  // rest > 0 implies sb = KB
+                hipLaunchKernelGGL(inner_update_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(m - sb, IU_ROWS))),
+                                   dim3(256), 0, s, rest, m - sb, P0, lda, (const T *)P0, ust, (int64_t)NB_OUTER);
+                LMX_LAUNCH_CHECK();
             }
         }
+        if (nbw > KB) {
+            hipLaunchKernelGGL(inner_flush_kernel<T>, dim3((unsigned)cdiv((int64_t)nbw * nbw, 256)), dim3(256), 0, s,
+                               nbw, stash, (int64_t)NB_OUTER, A + (int64_t)k0 * lda + k0, lda);
+            LMX_LAUNCH_CHECK();
+        }
         const int trail = N - (k0 + nbw);
+        T *A11 = A + (int64_t)k0 * lda + k0;
+        T *A12 = A11 + nbw;
+        T *W = Wall + (int64_t)(b & 1) * NB_OUTER * NB_OUTER;
+        const int narrow = std::min(NB_OUTER, trail);
+        if (pending) LMX_HIP(hipStreamWaitEvent(s, evF, 0));
         if (trail > 0) {
-            T *A11 = A + (int64_t)k0 * lda + k0;
-            T *A12 = A11 + nbw;
-            trsm_left<T>(s, true, false, true, nbw, trail, A11, lda, A12, lda);
-            gemm(s, false, false, trail, trail, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12, lda, T(1),
+            gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
+            tri_inverse_build<T>(s, true, false, true, nbw, A11, lda, W);
+            tri_inverse_solve<T>(s, true, false, nbw, narrow, A11, lda, W, A12, lda);
+            gemm(s, false, false, trail, narrow, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12, lda, T(1),
                  A12 + (int64_t)nbw * lda, lda);
         }
+        LMX_HIP(hipEventRecord(evE, s));
+        LMX_HIP(hipStreamWaitEvent(aux, evE, 0));
+        gather_rows_deferred<T>(aux, A, lda, 0, k0, pairs, np);
+        if (trail > narrow) {
+            tri_inverse_solve<T>(aux, true, false, nbw, trail - narrow, A11, lda, W, A12 + narrow, lda);
+            gemm(aux, false, false, trail, trail - narrow, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12 + narrow,
+                 lda, T(1), A12 + (int64_t)nbw * lda + narrow, lda);
+        }
+        LMX_HIP(hipEventRecord(evF, aux));
+        pending = true;
     }
+    if (pending) LMX_HIP(hipStreamWaitEvent(s, evF, 0));
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, s, &sy->error, d_info);
     LMX_LAUNCH_CHECK();
 }

@@ -12,8 +12,11 @@
 // redone with the FP64 factorisation (fell_back = 1).  Only eta (one double)
 // crosses to the host per step.
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 
 #include "common.hpp"
+#include "device.hpp"
 #include "kernels.hpp"
 
 namespace lmx {
@@ -60,18 +63,36 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
     atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
-// ||A||_inf = max_i sum_j |a_ij| : one workgroup per row block of 64 rows.
+// ||A||_inf = max_i sum_j |a_ij|: one wave per row at a time (coalesced
+// row reads, four partial sums per lane), rows strided over the grid, one
+// atomic per workgroup.
 __global__ __launch_bounds__(256) void norm_inf_rows_kernel(const double *__restrict__ A, int64_t lda,
                                                             int64_t n, double *__restrict__ out) {
-    __shared__ double part[4][64];
-    const int t = threadIdx.x, r = t & 63, q = t >> 6;
-    const int64_t i = (int64_t)blockIdx.x * 64 + r;
-    double s = 0.0;
-    if (i < n)
-        for (int64_t j = q; j < n; j += 4) s += fabs(A[i * lda + j]);
-    part[q][r] = s;
+    __shared__ double part[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    double best = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < n; i += nw) {
+        const double *row = A + i * lda;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int64_t j = lane;
+        for (; j + 192 < n; j += 256) {
+            s0 += fabs(row[j]);
+            s1 += fabs(row[j + 64]);
+            s2 += fabs(row[j + 128]);
+            s3 += fabs(row[j + 192]);
+        }
+        for (; j < n; j += 64) s0 += fabs(row[j]);
+        const double sum = wave_sum((s0 + s1) + (s2 + s3));
+        best = (sum > best || isnan(sum)) ? sum : best;
+    }
+    if (lane == 0) part[wave] = best;
     __syncthreads();
-    if (q == 0 && i < n) atomic_max_nonneg(out, part[0][r] + part[1][r] + part[2][r] + part[3][r]);
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int x = 1; x < 4; ++x) m = (part[x] > m || isnan(part[x])) ? part[x] : m;
+        atomic_max_nonneg(out, m);
+    }
 }
 
 // out[c] = max_i |X[i][c]| for an n x ncols block: a workgroup covers
@@ -98,6 +119,40 @@ __global__ __launch_bounds__(256) void colmax_kernel(const double *__restrict__ 
             if (!(part[x][cl] <= r)) r = part[x][cl];
         atomic_max_nonneg(&out[c], r);
     }
+}
+
+// The same for at most 8 columns: one thread per row (16 rows per thread,
+// coalesced), wave and workgroup reductions, one atomic per column and
+// workgroup.  NaN becomes +inf first (as the atomic would make it).
+constexpr int COLMAX_FEW = 8;
+__global__ __launch_bounds__(256) void colmax_few_kernel(const double *__restrict__ X, int64_t ldx, int64_t n,
+                                                         int ncols, double *__restrict__ out) {
+    __shared__ double part[4][COLMAX_FEW];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    double m[COLMAX_FEW];
+#pragma unroll
+    for (int c = 0; c < COLMAX_FEW; ++c) m[c] = 0.0;
+    const int64_t base = (int64_t)blockIdx.x * 4096;
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+        const int64_t i = base + k * 256 + t;
+        if (i < n) {
+#pragma unroll
+            for (int c = 0; c < COLMAX_FEW; ++c)
+                if (c < ncols) {
+                    double v = fabs(X[i * ldx + c]);
+                    if (isnan(v)) v = __builtin_huge_val();
+                    m[c] = fmax(m[c], v);
+                }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < COLMAX_FEW; ++c) {
+        const double w = wave_max(m[c]);
+        if (lane == 0) part[wave][c] = w;
+    }
+    __syncthreads();
+    if (t < ncols) atomic_max_nonneg(&out[t], fmax(fmax(part[0][t], part[1][t]), fmax(part[2][t], part[3][t])));
 }
 
 // R = B - A X in FP64 for nrhs <= NR: one wave per row of A (coalesced
@@ -138,6 +193,11 @@ __global__ __launch_bounds__(256) void resid_kernel(int64_t n, int nrhs, const d
 }
 
 void colmax(hipStream_t s, const double *X, int64_t ldx, int64_t n, int ncols, double *out) {
+    if (ncols <= COLMAX_FEW) {
+        hipLaunchKernelGGL(colmax_few_kernel, dim3((unsigned)cdiv(n, 4096)), dim3(256), 0, s, X, ldx, n, ncols, out);
+        LMX_LAUNCH_CHECK();
+        return;
+    }
     dim3 g((unsigned)cdiv(n, COLMAX_ROWS), (unsigned)cdiv(ncols, 64));
     hipLaunchKernelGGL(colmax_kernel, g, dim3(256), 0, s, X, ldx, n, ncols, out);
     LMX_LAUNCH_CHECK();
@@ -211,6 +271,16 @@ void low_factor_bind(LowFactor &f, void *mem) {
     f.WU = w ? reinterpret_cast<float *>(p + a32 + 256 + ints + w) : nullptr;
 }
 
+// LMX_LU_PIVOTING=tournament selects tournament pivoting for the FP32 LU
+// factor (read once per process).
+bool tournament_pivoting() {
+    static const bool on = [] {
+        const char *e = std::getenv("LMX_LU_PIVOTING");
+        return e != nullptr && std::strcmp(e, "tournament") == 0;
+    }();
+    return on;
+}
+
 void low_factor(hipStream_t s, LowFactor &f) {
     const int64_t n = f.n;
     launch1d(s, n * n, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, f.A, f.lda, f.A32, n, n, n); });
@@ -218,7 +288,7 @@ void low_factor(hipStream_t s, LowFactor &f) {
     if (f.spd)
         potrf<float>(s, n, f.A32, n, f.dinfo);
     else
-        getrf_impl<float>(s, n, f.A32, n, f.ipiv, f.perm, f.dinfo);
+        getrf_impl<float>(s, n, f.A32, n, f.ipiv, f.perm, f.dinfo, tournament_pivoting());
     // inverses of the factor's diagonal blocks for the refinement solves (a
     // failed factor leaves them unused)
     if (f.WL) {
@@ -231,7 +301,8 @@ void low_factor(hipStream_t s, LowFactor &f) {
         }
     }
     hipLaunchKernelGGL(zero_d_kernel, dim3(1), dim3(256), 0, s, f.anorm, 1);
-    hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, f.A, f.lda, n, f.anorm);
+    hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 4), 1024)), dim3(256), 0, s, f.A,
+                       f.lda, n, f.anorm);
     LMX_LAUNCH_CHECK();
     int32_t hinfo = 0;
     LMX_HIP(hipMemcpyAsync(&hinfo, f.dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -314,7 +385,8 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
         // report the backward error of the FP64 solve
         residual(s, n, k, A, lda, X, ldx, B, ldb, R, k);
         hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
-        hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)cdiv(n, 64)), dim3(256), 0, s, A, lda, n, norms);
+        hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 4), 1024)), dim3(256), 0, s,
+                           A, lda, n, norms);
         colmax(s, B, ldb, n, K, norms + 1);
         colmax(s, X, ldx, n, K, norms + 1 + K);
         colmax(s, R, k, n, K, norms + 1 + 2 * K);

@@ -25,13 +25,14 @@ namespace {
 
 constexpr int TB = 64;
 
-// A diagonal block of M = op(T) in LDS as a LOWER triangle of order S:
-// M restricted to its triangle and padded with the identity beyond nb (nb
-// <= 0: the identity); an upper M is staged reversed, Ms[r][c] =
-// M[S-1-r][S-1-c] (J M J with J the reversal is lower), so one forward
-// solve serves both.  Thread t reads column t of each stored row i --
-// coalesced -- which is M[i][t], or M[t][i] for a transposed operand.
-// Rows are 16-byte aligned so the solve reads four coefficients per access.
+// A diagonal block of M = op(T) in LDS as a LOWER triangle L of order S,
+// stored by columns (Ms[c][r] = L[r][c]): M restricted to its triangle and
+// padded with the identity beyond nb (nb <= 0: the identity); an upper M
+// is staged reversed, L[r][c] = M[S-1-r][S-1-c] (J M J with J the reversal
+// is lower), so one forward solve serves both.  Thread t reads column t of
+// each stored row i -- coalesced -- which is M[i][t], or M[t][i] for a
+// transposed operand.  Rows of Ms are 16-byte aligned so the solve reads
+// four coefficients per access.
 constexpr int ldm_of(int bytes) { return TB + 16 / bytes; }
 template <class T>
 __device__ inline void stage_block(T (*Ms)[ldm_of(sizeof(T))], const T *A, int64_t lda, int nb, int t, bool mt,
@@ -49,24 +50,25 @@ __device__ inline void stage_block(T (*Ms)[ldm_of(sizeof(T))], const T *A, int64
             v = r == c ? T(1) : T(0);
         }
         if (mlower)
-            Ms[r][c] = v;
+            Ms[c][r] = v;
         else
-            Ms[S - 1 - r][S - 1 - c] = v;
+            Ms[S - 1 - c][S - 1 - r] = v;
     }
 }
 
-// L x = b in place on the registers x[0..S), L lower in LDS: row form
-// x_r = (b_r - sum_{k<r} l_rk x_k) / l_rr, k ascending (for a reversed
-// upper block: the reference back substitution with its sum taken from
-// the far end).  A unit diagonal is staged as 1; dividing by 1 is exact.
+// L x = b in place on the registers x[0..S), L lower in LDS by columns:
+// x_r = (b_r - sum_{k<r} l_rk x_k) / l_rr with the terms subtracted in
+// ascending k (for a reversed upper block: the reference back substitution
+// with its sum taken from the far end) -- evaluated column by column, so
+// that the updates of one step are independent and the dependent path is S
+// steps, not S^2/2.  A unit diagonal is staged as 1; dividing by 1 is exact.
 template <class T, int S>
 __device__ __forceinline__ void solve_regs(const T (*Ms)[ldm_of(sizeof(T))], T (&x)[TB]) {
 #pragma unroll
-    for (int r = 0; r < S; ++r) {
-        T s = x[r];
+    for (int k = 0; k < S; ++k) {
+        x[k] = x[k] / Ms[k][k];
 #pragma unroll
-        for (int k = 0; k < r; ++k) s = fma(-Ms[r][k], x[k], s);
-        x[r] = s / Ms[r][r];
+        for (int r = k + 1; r < S; ++r) x[r] = fma(-Ms[k][r], x[k], x[r]);
     }
 }
 
