@@ -234,11 +234,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
 template <class T, int BM, int BN, int BK>
 void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, const T *A,
                  int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0,
-                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}) {
+                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}, size_t reserve_lds = 0) {
     int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
     dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n, (unsigned)batch), block(256);
 #define LMX_GEMM_CASE(TA_, TB_)                                                              \
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, BK, TA_, TB_>), grid, block, 0, s, m, n, k, \
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, BK, TA_, TB_>), grid, block, reserve_lds, s, m, n, k, \
                        alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, lower, bs)
     if (!ta && !tb)
         LMX_GEMM_CASE(false, false);
@@ -254,7 +254,7 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
 
 template <class T>
 void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
-               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc) {
+               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, size_t reserve_lds = 0) {
     if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
     if (m == 0 || n == 0) return;
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
@@ -266,11 +266,11 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     if (big_tiles >= 512) {
         TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, flops);
         launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
-                                     C, ldc);
+                                     C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, reserve_lds);
     } else {
         TimedRegion tr(sizeof(T) == 8 ? "gemm64" : "sgemm64", s, flops);
         launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
-                                   C, ldc);
+                                   C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, reserve_lds);
     }
 }
 
@@ -324,6 +324,16 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
           int64_t ldc) {
     gemm_impl<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
+
+template <class T>
+void gemm_sharing(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
+                  int64_t ldb, T beta, T *C, int64_t ldc, size_t reserve_lds) {
+    gemm_impl<T>(s, false, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, reserve_lds);
+}
+template void gemm_sharing<double>(hipStream_t, int64_t, int64_t, int64_t, double, const double *, int64_t,
+                                   const double *, int64_t, double, double *, int64_t, size_t);
+template void gemm_sharing<float>(hipStream_t, int64_t, int64_t, int64_t, float, const float *, int64_t,
+                                  const float *, int64_t, float, float *, int64_t, size_t);
 
 template <class T>
 void gemm_batched(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
