@@ -87,6 +87,51 @@ void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, in
     }
 }
 
+// The top level with lookahead: the caller's stream factors block column
+// b + 1 (diagonal block, then the column below it) while an auxiliary
+// stream applies block b's SYRK to the columns beyond b + 1.
+//   stream s:   panel(b); wait done(b-1); b's update of block column b + 1
+//               (SYRK of its diagonal block, GEMM below it); record ready(b)
+//   stream aux: wait ready(b); SYRK of the trailing square beyond b + 1;
+//               record done(b)
+// While they overlap the streams write disjoint columns (b + 1's against
+// those beyond it), and the auxiliary stream only reads block column b.
+template <class T>
+void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, int64_t nb) {
+    AuxStream la;
+    bool pending = false;
+    for (int64_t k0 = 0; k0 < n; k0 += nb) {
+        const int64_t kb = std::min(nb, n - k0);
+        T *A11 = A + k0 * lda + k0;
+        if (kb <= DIAG) {
+            TimedRegion tr("potrf_diag", s);
+            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)kb, A11, lda, (int)k0, d_info);
+            LMX_LAUNCH_CHECK();
+        } else {
+            potrf_blocked<T>(s, kb, A11, lda, k0, d_info, DIAG);
+        }
+        const int64_t rest = n - k0 - kb;
+        T *A21 = A11 + kb * lda;
+        if (rest > 0) trsm_right<T>(s, true, true, false, rest, kb, A11, lda, A21, lda);
+        if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
+        const int64_t narrow = std::min(nb, rest);
+        if (rest > 0) {
+            syrk_lower(s, narrow, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
+            if (rest > narrow)
+                gemm(s, false, true, rest - narrow, narrow, kb, T(-1), A21 + narrow * lda, lda, A21, lda, T(1),
+                     A21 + narrow * lda + kb, lda);
+        }
+        LMX_HIP(hipEventRecord(la.ready, s));
+        LMX_HIP(hipStreamWaitEvent(la.aux, la.ready, 0));
+        if (rest > narrow)
+            syrk_lower(la.aux, rest - narrow, kb, T(-1), A21 + narrow * lda, lda, T(1), A21 + narrow * lda + kb + narrow,
+                       lda);
+        LMX_HIP(hipEventRecord(la.done, la.aux));
+        pending = true;
+    }
+    if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
+}
+
 }  // namespace
 
 template <class T>
@@ -96,7 +141,7 @@ void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info) {
     clear_info(s, d_info);
     if (n == 0) return;
     TimedRegion trall("potrf", s, (double)n * (double)n * (double)n / 3.0);
-    potrf_blocked<T>(s, n, A, lda, 0, d_info, 128);
+    potrf_lookahead<T>(s, n, A, lda, d_info, 128);
 }
 
 template <class T>

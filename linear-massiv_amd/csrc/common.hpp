@@ -84,6 +84,19 @@ struct Scratch {
     T *as() const { return static_cast<T *>(p); }
 };
 
+// A second stream and two events for a factorisation's lookahead: the
+// caller's stream records `ready` when a block's update may start, the
+// auxiliary stream waits for it, runs the update and records `done`.
+// Destroyed (stream-ordered) when the factorisation returns.
+struct AuxStream {
+    hipStream_t aux = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    AuxStream();
+    ~AuxStream();
+    AuxStream(const AuxStream &) = delete;
+    AuxStream &operator=(const AuxStream &) = delete;
+};
+
 // Optional per-family kernel timing (lmx_kernel_timing).
 bool timing_enabled();
 // `work` is the algorithmic work of the region (flops, or bytes for

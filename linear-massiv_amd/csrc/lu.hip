@@ -1063,20 +1063,9 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
     //               U12 and A22 -= L21 U12 on panel b+1's columns; record E(b)
     //   stream aux: wait E(b); swaps(b) on [0, k0); U12 and A22 -= L21 U12 on
     //               the columns beyond panel b+1; record F(b)
-    hipStream_t aux = nullptr;
-    hipEvent_t evE = nullptr, evF = nullptr;
-    LMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
-    LMX_HIP(hipEventCreateWithFlags(&evE, hipEventDisableTiming));
-    LMX_HIP(hipEventCreateWithFlags(&evF, hipEventDisableTiming));
-    struct Cleanup {
-        hipStream_t a;
-        hipEvent_t e, f;
-        ~Cleanup() {
-            if (e) (void)hipEventDestroy(e);
-            if (f) (void)hipEventDestroy(f);
-            if (a) (void)hipStreamDestroy(a);
-        }
-    } cleanup{aux, evE, evF};
+    AuxStream la;
+    hipStream_t aux = la.aux;
+    hipEvent_t evE = la.ready, evF = la.done;
     bool pending = false;  // evF marks the auxiliary stream's last block
     unsigned base = 0;
     for (int b = 0, k0 = 0; k0 < N; ++b, k0 += NB_OUTER) {

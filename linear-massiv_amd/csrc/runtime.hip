@@ -101,6 +101,18 @@ Scratch::~Scratch() {
     if (p) (void)hipFreeAsync(p, s);
 }
 
+AuxStream::AuxStream() {
+    LMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    LMX_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    LMX_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+}
+
+AuxStream::~AuxStream() {
+    if (ready) (void)hipEventDestroy(ready);
+    if (done) (void)hipEventDestroy(done);
+    if (aux) (void)hipStreamDestroy(aux);
+}
+
 bool timing_enabled() { return g_timing; }
 
 TimedRegion::TimedRegion(const char *family, hipStream_t st, double w) : s(st), work(w) {
