@@ -97,6 +97,12 @@ int lmx_sgemm(char transa, char transb, int64_t m, int64_t n, int64_t k,
 int lmx_dgetrf(int64_t n, double *A, int64_t lda, int32_t *ipiv);
 int lmx_dgetrf_dev(void *stream, int64_t n, double *dA, int64_t lda,
                    int32_t *d_ipiv, int32_t *d_info);
+/* The same factorisation in FP32: lu at e = Float (the reference's lu is
+ * generic over Fractional e, src/.../Solve/LU.hs:121).  This is the factor
+ * the mixed-precision solvers refine with. */
+int lmx_sgetrf(int64_t n, float *A, int64_t lda, int32_t *ipiv);
+int lmx_sgetrf_dev(void *stream, int64_t n, float *dA, int64_t lda,
+                   int32_t *d_ipiv, int32_t *d_info);
 /* Solve A X = B with the factors of lmx_dgetrf; B (n x nrhs) is
  * overwritten by X.  Replaces the solve phase of luSolveP
  * (src/.../Solve/LU.hs:294-303: applyPivotsForward,

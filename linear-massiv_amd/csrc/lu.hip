@@ -1214,6 +1214,25 @@ int read_info(hipStream_t s, const int32_t *d_info) {
     return info;
 }
 
+template <class T>
+int getrf_host(int64_t n, T *A, int64_t lda, int32_t *ipiv) {
+    return guarded([&] {
+        check_square(n, lda, "getrf");
+        if (n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda) / sizeof(double) * sizeof(T);
+        T *dA = (T *)staging(c, 0, ab);
+        int32_t *dp = (int32_t *)staging(c, 1, (size_t)n * 4 + 64);
+        int32_t *di = dp + n;
+        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        getrf<T>(c.stream, n, dA, lda, dp, di);
+        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipMemcpyAsync(ipiv, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+        return read_info(c.stream, di);
+    });
+}
+
 }  // namespace
 
 extern "C" {
@@ -1249,21 +1268,15 @@ int lmx_dgesv_dev(void *stream, int64_t n, int64_t nrhs, double *dA, int64_t lda
     });
 }
 
-int lmx_dgetrf(int64_t n, double *A, int64_t lda, int32_t *ipiv) {
+int lmx_dgetrf(int64_t n, double *A, int64_t lda, int32_t *ipiv) { return getrf_host<double>(n, A, lda, ipiv); }
+
+int lmx_sgetrf(int64_t n, float *A, int64_t lda, int32_t *ipiv) { return getrf_host<float>(n, A, lda, ipiv); }
+
+int lmx_sgetrf_dev(void *stream, int64_t n, float *dA, int64_t lda, int32_t *d_ipiv, int32_t *d_info) {
     return guarded([&] {
         check_square(n, lda, "getrf");
-        if (n == 0) return LMX_OK;
-        Context &c = ctx();
-        std::lock_guard<std::mutex> g(c.host_mu);
-        size_t ab = extent(n, n, lda);
-        double *dA = (double *)staging(c, 0, ab);
-        int32_t *dp = (int32_t *)staging(c, 1, (size_t)n * 4 + 64);
-        int32_t *di = dp + n;
-        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
-        getrf<double>(c.stream, n, dA, lda, dp, di);
-        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
-        LMX_HIP(hipMemcpyAsync(ipiv, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
-        return read_info(c.stream, di);
+        getrf<float>(resolve_stream(stream), n, dA, lda, d_ipiv, d_info);
+        return LMX_OK;
     });
 }
 

@@ -55,7 +55,7 @@ PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 EXPORTED = [
     "lmx_init", "lmx_finalize", "lmx_last_error", "lmx_version",
     "lmx_dgemm", "lmx_dgemm_dev", "lmx_sgemm", "lmx_sgemm_dev",
-    "lmx_dgetrf", "lmx_dgetrf_dev", "lmx_dgetrs", "lmx_dgetrs_dev", "lmx_dgesv", "lmx_dgesv_dev",
+    "lmx_dgetrf", "lmx_dgetrf_dev", "lmx_sgetrf", "lmx_sgetrf_dev", "lmx_dgetrs", "lmx_dgetrs_dev", "lmx_dgesv", "lmx_dgesv_dev",
     "lmx_dpotrf", "lmx_dpotrf_dev", "lmx_dpotrs", "lmx_dpotrs_dev", "lmx_dposv", "lmx_dposv_dev",
     "lmx_dtrsm", "lmx_dtrsm_dev",
     "lmx_dgesv_mixed", "lmx_dgesv_mixed_dev", "lmx_dposv_mixed", "lmx_dposv_mixed_dev",
@@ -122,6 +122,8 @@ _sigs = {
     "lmx_sgemm_dev": [_vp, _chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
     "lmx_dgetrf": [_i64, _vp, _i64, _vp],
     "lmx_dgetrf_dev": [_vp, _i64, _vp, _i64, _vp, _vp],
+    "lmx_sgetrf": [_i64, _vp, _i64, _vp],
+    "lmx_sgetrf_dev": [_vp, _i64, _vp, _i64, _vp, _vp],
     "lmx_dgetrs": [_i64, _i64, _vp, _i64, _vp, _vp, _i64],
     "lmx_dgetrs_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64],
     "lmx_dgesv": [_i64, _i64, _vp, _i64, _vp, _vp, _i64],
@@ -247,6 +249,15 @@ def getrf(A):
     n = A.shape[0]
     ipiv = np.zeros(max(n, 1), np.int32)
     info = _check(_lib.lmx_dgetrf(n, _ptr(A), max(n, 1), _ptr(ipiv)), allow_info=True)
+    return A, ipiv[:n], info
+
+
+def sgetrf(A):
+    """FP32 LU: -> (packed LU as float32, ipiv swap list (0-based), info)."""
+    A = np.array(A, np.float32, order="C", copy=True)
+    n = A.shape[0]
+    ipiv = np.zeros(max(n, 1), np.int32)
+    info = _check(_lib.lmx_sgetrf(n, _ptr(A), max(n, 1), _ptr(ipiv)), allow_info=True)
     return A, ipiv[:n], info
 
 
@@ -429,6 +440,10 @@ def sgemm_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", stream=None)
 
 def getrf_dev(A, ipiv, info, stream=None):
     _check(_lib.lmx_dgetrf_dev(_stream(stream), A.shape[0], _tp(A), A.stride(0), _tp(ipiv), _tp(info)))
+
+
+def sgetrf_dev(A, ipiv, info, stream=None):
+    _check(_lib.lmx_sgetrf_dev(_stream(stream), A.shape[0], _tp(A), A.stride(0), _tp(ipiv), _tp(info)))
 
 
 def getrs_dev(LU, ipiv, B, stream=None):

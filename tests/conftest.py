@@ -35,3 +35,34 @@ def lmx_gpu():
     import lmx
     lmx.init(0)
     return lmx
+
+
+def known_pivot_matrix(n, seed):
+    """-> (A, L, U, sigma) with A[sigma[j]] = row j of L U, where L is unit
+    lower with |l_ij| <= 1/2 and 1 <= |u_jj| <= 2.  At step j of partial
+    pivoting the candidates are l_ij u_jj, so row j of L U wins by a factor
+    of two in exact arithmetic; the multipliers and the off-diagonal part of
+    U shrink like 1/sqrt(i), which keeps L and U well conditioned, so FP32
+    rounding cannot close that margin."""
+    rng = np.random.default_rng(seed)
+    scale = 0.5 / np.sqrt(np.maximum(np.arange(n), 1))
+    L = np.tril(rng.uniform(-1, 1, (n, n)) * scale[:, None], -1) + np.eye(n)
+    U = np.triu(rng.uniform(-1, 1, (n, n)) * scale[None, :], 1)
+    U += np.diag(rng.choice([-1.0, 1.0], n) * rng.uniform(1, 2, n))
+    sigma = rng.permutation(n)
+    A = np.empty((n, n))
+    A[sigma] = L @ U
+    return A, L, U, sigma
+
+
+def swaps_to_perm(ipiv):
+    """LAPACK swap list (0-based) -> permutation: row i of P A is row perm[i] of A."""
+    perm = np.arange(len(ipiv))
+    for k, p in enumerate(ipiv):
+        perm[[k, p]] = perm[[p, k]]
+    return perm
+
+
+@pytest.fixture(scope="session")
+def known_pivots():
+    return known_pivot_matrix
