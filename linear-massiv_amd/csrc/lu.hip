@@ -705,18 +705,26 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
 // (rows 32 + 64 b..) takes A22 -= L21 U12.  U12 goes to `ust` (row stride ldst), not into
 // B12: other workgroups may still be reading B12; the driver copies it into
 // place once the outer panel is done (nothing reads it before).
-constexpr int IU_ROWS = 64;
+// Rows per workgroup: 32 in FP32, so that the kernel's LDS (the tile of L21
+// and U12) fits beside two workgroups of the FP32 lookahead GEMM on a CU
+// (it runs while that GEMM does; with 64 rows it would wait for GEMM tiles
+// to retire).
+template <class T>
+constexpr int iu_rows() {
+    return sizeof(T) == 4 ? 32 : 64;
+}
 constexpr int IU_MAXREST = NB_OUTER - KB;
 template <class T>
 __global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, T *__restrict__ P0, int64_t lda,
                                                            const T *__restrict__ L11, T *__restrict__ ust,
                                                            int64_t ldst) {
     constexpr int kb = KB;
-    __shared__ __attribute__((aligned(16))) T L2[IU_ROWS][KB];
+    constexpr int ROWS = iu_rows<T>(), RG = ROWS / 4;  // rows per workgroup, per group of 64 threads
+    __shared__ __attribute__((aligned(16))) T L2[ROWS][KB];
     __shared__ T Us[KB][IU_MAXREST + 1];
     const int t = threadIdx.x;
-    const int64_t rt = (int64_t)kb + (int64_t)blockIdx.x * IU_ROWS;  // first tile row, panel-relative
-    for (int e = t; e < IU_ROWS * KB; e += 256) {
+    const int64_t rt = (int64_t)kb + (int64_t)blockIdx.x * ROWS;  // first tile row, panel-relative
+    for (int e = t; e < ROWS * KB; e += 256) {
         const int i = e / KB, k = e % KB;
         L2[i][k] = (rt + i < kb + mrows && k < kb) ? P0[(rt + i) * lda + k] : T(0);
     }
@@ -724,17 +732,17 @@ __global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, 
     const T *B12 = P0 + kb;
 #pragma unroll
     for (int i = 0; i < KB; ++i) x[i] = (t < rest && i < kb) ? B12[(int64_t)i * lda + t] : T(0);
-    // this thread's A22 elements (column cl + 64 j, rows g*16 + r of the
+    // this thread's A22 elements (column cl + 64 j, rows g*RG + r of the
     // tile), loaded now so their latency overlaps the solve
     const int g = t >> 6, cl = t & 63;
     constexpr int NJ = (IU_MAXREST + 63) / 64;
-    T pre[NJ][16];
+    T pre[NJ][RG];
 #pragma unroll
     for (int j = 0; j < NJ; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+        for (int r = 0; r < RG; ++r) {
             const int c = cl + 64 * j;
-            const int64_t row = rt + g * 16 + r;
+            const int64_t row = rt + g * RG + r;
             pre[j][r] = (c < rest && row < kb + mrows) ? P0[row * lda + kb + c] : T(0);
         }
     __syncthreads();
@@ -753,20 +761,20 @@ __global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, 
             for (int i = 0; i < kb; ++i) ust[(int64_t)i * ldst + t] = x[i];
     }
     __syncthreads();
-    // A22 tile -= L21 U12: thread (g, c) takes 16 rows of column c
+    // A22 tile -= L21 U12: thread (g, c) takes RG rows of column c
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
         const int c = cl + 64 * j;
         if (c >= rest) break;
-        T acc[16];
+        T acc[RG];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = T(0);
+        for (int r = 0; r < RG; ++r) acc[r] = T(0);
 #pragma unroll 2
         for (int k = 0; k < KB; k += 4) {
             const T u0 = Us[k][c], u1 = Us[k + 1][c], u2 = Us[k + 2][c], u3 = Us[k + 3][c];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const T *l = &L2[g * 16 + r][k];
+            for (int r = 0; r < RG; ++r) {
+                const T *l = &L2[g * RG + r][k];
                 acc[r] = fma(l[0], u0, acc[r]);
                 acc[r] = fma(l[1], u1, acc[r]);
                 acc[r] = fma(l[2], u2, acc[r]);
@@ -774,8 +782,8 @@ __global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, 
             }
         }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t row = rt + g * 16 + r;
+        for (int r = 0; r < RG; ++r) {
+            const int64_t row = rt + g * RG + r;
             if (row < kb + mrows) P0[row * lda + kb + c] = pre[j][r] - acc[r];
         }
     }
@@ -1092,7 +1100,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
                 // U12 into the stash at (s0 - k0, s0 - k0 + sb) of the outer block
                 T *ust = stash + (int64_t)(s0 - k0) * NB_OUTER + (s0 - k0) + sb;
                 if (sb != KB) fail_arg("getrf: inner update needs a full sub-panel");  // rest > 0 implies sb = KB
-                hipLaunchKernelGGL(inner_update_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(m - sb, IU_ROWS))),
+                hipLaunchKernelGGL(inner_update_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(m - sb, iu_rows<T>()))),
                                    dim3(256), 0, s, rest, m - sb, P0, lda, (const T *)P0, ust, (int64_t)NB_OUTER);
                 LMX_LAUNCH_CHECK();
             }
