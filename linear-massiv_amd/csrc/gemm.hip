@@ -7,6 +7,9 @@
 // step's global loads are issued into registers before the current step's
 // MFMAs and written to the other buffer after them, one barrier per step.
 //
+// The NN case (both operands as stored) with 16-byte aligned rows has its
+// own kernel, gemm_nn_kernel, described there.
+//
 // MFMA 16x16x4 operands (f64 and f32 alike): lane l supplies
 // A[i = l%16][kk = l/16] and B[kk = l/16][j = l%16], so both operand tiles
 // live in LDS k-major (As[kk][i], Bs[kk][j]) and the 16 lanes of a
@@ -22,6 +25,21 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// 16-byte vectors of T (the width of one global or LDS access)
+template <class T>
+struct V16;
+template <>
+struct V16<float> {
+    typedef f4 type;
+    static constexpr int N = 4;
+};
+template <>
+struct V16<double> {
+    typedef d2 type;
+    static constexpr int N = 2;
+};
 
 template <class T>
 struct Acc;
@@ -231,12 +249,191 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
         }
 }
 
+// C = alpha A B + beta C with A (m x k) and B (k x n) as stored, every row
+// 16-byte aligned and k, n multiples of the 16-byte vector width.
+//   Global loads are 16-byte vectors.  A tiles go to LDS as they are
+//   (As[i][k]); B tiles are transposed on the way in (Bs[j][k]), the lanes of
+//   a wave on consecutive k so that the transposing writes do not collide.
+//   MFMA step s of a 16-k chunk takes k = 4 (lane / 16) + s from lane group
+//   lane / 16 (the same products as steps of consecutive k, summed in
+//   another order), so a lane's operands for the whole chunk are four
+//   consecutive elements of one LDS row: one 128-bit read per 16 x 16 block
+//   for FP32 (two for FP64) instead of four scalar reads.  The row stride
+//   (20 floats, 18 doubles) puts the 128-bit reads of 8 consecutive rows on
+//   distinct banks.
+template <class T, int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alpha, const T *__restrict__ A,
+                                                      int64_t lda, const T *__restrict__ B, int64_t ldb, T beta,
+                                                      T *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
+                                                      BatchStrides bs) {
+    constexpr int BK = 16;
+    constexpr int SK = sizeof(T) == 4 ? BK + 4 : BK + 2;
+    typedef typename V16<T>::type VT;
+    constexpr int E = V16<T>::N;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int MT = WM / 16, NT = WN / 16;
+    constexpr int LA = BM * BK / (E * 256);  // A vectors loaded per thread per step
+    constexpr int LB = BN * BK / (E * 256);
+    static_assert(LA >= 1 && LB >= 1 && BM * BK % (E * 256) == 0 && BN * BK % (E * 256) == 0, "tile shape");
+    typedef typename Acc<T>::type V;
+
+    __shared__ __attribute__((aligned(16))) T As[2][BM][SK];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BN][SK];
+
+    if (gridDim.y > 1) {
+        const int zk = (int)blockIdx.y / bs.nb1, zp = (int)blockIdx.y - zk * bs.nb1;
+        A += zk * bs.a2 + zp * bs.a1;
+        B += zk * bs.b2 + zp * bs.b1;
+        C += zk * bs.c2 + zp * bs.c1;
+    }
+    int tm, tn;
+    tile_of(tiles_m, tiles_n, false, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    V acc[MT][NT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) acc[a][b] = V{0, 0, 0, 0};
+
+    VT ra[LA], rb[LB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int e = it * 256 + tid;
+            const int i = e / (BK / E), kq = e % (BK / E);
+            const int gi = m0 + i, gk = k0 + kq * E;
+            ra[it] = (gi < m && gk < k) ? *reinterpret_cast<const VT *>(A + (int64_t)gi * lda + gk) : VT{};
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int e = it * 256 + tid;
+            const int kk = e % BK, jq = e / BK;
+            const int gj = n0 + jq * E, gk = k0 + kk;
+            rb[it] = (gk < k && gj < n) ? *reinterpret_cast<const VT *>(B + (int64_t)gk * ldb + gj) : VT{};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int e = it * 256 + tid;
+            const int i = e / (BK / E), kq = e % (BK / E);
+            *reinterpret_cast<VT *>(&As[buf][i][kq * E]) = ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int e = it * 256 + tid;
+            const int kk = e % BK, jq = e / BK;
+#pragma unroll
+            for (int q = 0; q < E; ++q) Bs[buf][jq * E + q][kk] = rb[it][q];
+        }
+    };
+
+    const int KT = (k + BK - 1) / BK;
+    if (KT > 0) {
+        load(0);
+        store(0);
+        __syncthreads();
+    }
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < KT;
+        if (more) load((kt + 1) * BK);
+        T fa[MT][4], fb[NT][4];
+#pragma unroll
+        for (int a = 0; a < MT; ++a) {
+            const T *p = &As[cur][wm0 + a * 16 + lr][4 * lk];
+#pragma unroll
+            for (int v = 0; v < 4 / E; ++v) {
+                const VT x = *reinterpret_cast<const VT *>(p + v * E);
+#pragma unroll
+                for (int q = 0; q < E; ++q) fa[a][v * E + q] = x[q];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            const T *p = &Bs[cur][wn0 + b * 16 + lr][4 * lk];
+#pragma unroll
+            for (int v = 0; v < 4 / E; ++v) {
+                const VT x = *reinterpret_cast<const VT *>(p + v * E);
+#pragma unroll
+                for (int q = 0; q < E; ++q) fb[b][v * E + q] = x[q];
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+            for (int a = 0; a < MT; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b) acc[a][b] = mfma16x16x4(fa[a][st], fb[b][st], acc[a][b]);
+        if (more) store(cur ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            const int gj = n0 + wn0 + b * 16 + lr;
+            if (gj >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm0 + a * 16 + acc_row<T>(lk, r);
+                if (gi < m) {
+                    T *c = C + (int64_t)gi * ldc + gj;
+                    T v = alpha * acc[a][b][r];
+                    if (beta != T(0)) v += beta * *c;
+                    *c = v;
+                }
+            }
+        }
+}
+
+// LDS of one workgroup of the two kernels (static arrays)
+template <class T, int BM, int BN, int BK>
+constexpr size_t gemm_lds() {
+    return 2 * (size_t)BK * (BM + BN + 2 * ((sizeof(T) == 8) ? 1 : 2)) * sizeof(T);
+}
+template <class T, int BM, int BN>
+constexpr size_t gemm_nn_lds() {
+    return 2 * (size_t)(BM + BN) * (sizeof(T) == 4 ? 20 : 18) * sizeof(T);
+}
+
+// Whether gemm_nn_kernel applies: no transposes, 16-byte aligned rows, k and
+// n multiples of the vector width (a vector is then wholly inside or outside).
+template <class T>
+bool nn_ok(bool ta, bool tb, int64_t n, int64_t k, const T *A, int64_t lda, const T *B, int64_t ldb,
+           const BatchStrides &bs) {
+    constexpr int E = V16<T>::N;
+    auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+    return !ta && !tb && al(A) && al(B) && lda % E == 0 && ldb % E == 0 && k % E == 0 && n % E == 0 &&
+           bs.a1 % E == 0 && bs.a2 % E == 0 && bs.b1 % E == 0 && bs.b2 % E == 0;
+}
+
 template <class T, int BM, int BN, int BK>
 void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, const T *A,
                  int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0,
-                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}, size_t reserve_lds = 0) {
+                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}, size_t lds_footprint = 0) {
     int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
     dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n, (unsigned)batch), block(256);
+    // lds_footprint: the LDS one workgroup should hold in all (its tiles plus
+    // an unused reserve), so that fewer workgroups share a CU; 0 = no reserve.
+    // A GEMM that shares its CUs with FP32 panel workgroups keeps the first
+    // kernel: the NN kernel is faster on its own but slows those panels more
+    // (FP32 LU at n=16384: 155 against 151 ms).
+    const bool sharing = lds_footprint > 0 && sizeof(T) == 4;
+    if (!lower && !sharing && nn_ok<T>(ta, tb, n, k, A, lda, B, ldb, bs)) {
+        const size_t st = gemm_nn_lds<T, BM, BN>();
+        hipLaunchKernelGGL((gemm_nn_kernel<T, BM, BN>), grid, block, lds_footprint > st ? lds_footprint - st : 0, s,
+                           m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, bs);
+        LMX_LAUNCH_CHECK();
+        return;
+    }
+    const size_t st = gemm_lds<T, BM, BN, BK>(), reserve_lds = lds_footprint > st ? lds_footprint - st : 0;
 #define LMX_GEMM_CASE(TA_, TB_)                                                              \
     hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, BK, TA_, TB_>), grid, block, reserve_lds, s, m, n, k, \
                        alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, lower, bs)
@@ -254,7 +451,7 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
 
 template <class T>
 void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
-               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, size_t reserve_lds = 0) {
+               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint = 0) {
     if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
     if (m == 0 || n == 0) return;
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
@@ -266,11 +463,11 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     if (big_tiles >= 512) {
         TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, flops);
         launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
-                                     C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, reserve_lds);
+                                     C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
     } else {
         TimedRegion tr(sizeof(T) == 8 ? "gemm64" : "sgemm64", s, flops);
         launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
-                                   C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, reserve_lds);
+                                   C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
     }
 }
 
@@ -327,8 +524,8 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
 
 template <class T>
 void gemm_sharing(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
-                  int64_t ldb, T beta, T *C, int64_t ldc, size_t reserve_lds) {
-    gemm_impl<T>(s, false, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, reserve_lds);
+                  int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint) {
+    gemm_impl<T>(s, false, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, lds_footprint);
 }
 template void gemm_sharing<double>(hipStream_t, int64_t, int64_t, int64_t, double, const double *, int64_t,
                                    const double *, int64_t, double, double *, int64_t, size_t);

@@ -20,12 +20,13 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
           const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
           int64_t ldc);
 
-// C = alpha A B + beta C launched with reserve_lds bytes of extra LDS per
-// workgroup: fewer GEMM workgroups share a CU, which leaves room for the
-// workgroups of a kernel running beside it on another stream.
+// C = alpha A B + beta C with each workgroup holding lds_footprint bytes of
+// LDS in all (its tiles plus an unused reserve): fewer GEMM workgroups share
+// a CU, which leaves room for the workgroups of a kernel running beside it
+// on another stream.
 template <class T>
 void gemm_sharing(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
-                  int64_t ldb, T beta, T *C, int64_t ldc, size_t reserve_lds);
+                  int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint);
 
 // Batched C_z = alpha A_z B_z + beta C_z (no transposes) for z = zk nb1 + zp,
 // zk < nb2, zp < nb1; entry z's matrices start at zk x2 + zp x1 elements

@@ -861,13 +861,14 @@ __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int6
 // that another stream was still using.
 constexpr int PAIR_STRIDE = 4 * KB + 4;
 constexpr int PANELS_PER_BLOCK = NB_OUTER / KB;
-// Extra LDS for the lookahead's trailing GEMM: 30 KB keeps the FP32 GEMM
-// at two workgroups per CU (33 KB of tiles each), leaving a CU room for an
-// FP32 panel workgroup (21 KB); the FP64 GEMM tiles (67 KB) already leave
-// one workgroup's room, and a reserve would cost them half their occupancy.
+// LDS footprint of a workgroup of the lookahead's trailing GEMM: 64000
+// bytes keep the FP32 GEMM at two workgroups per CU, leaving a CU room for
+// an FP32 panel workgroup (21 KB) or inner-update workgroup (33 KB); the
+// FP64 GEMM tiles already hold that much, and a reserve would cost them
+// half their occupancy (0: none).
 template <class T>
-constexpr size_t aux_gemm_reserve_lds() {
-    return sizeof(T) == 4 ? 30 * 1024 : 0;
+constexpr size_t aux_gemm_lds_footprint() {
+    return sizeof(T) == 4 ? 64000 : 0;
 }
 template <class T>
 __global__ __launch_bounds__(256) void row_gather_multi_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
@@ -1131,7 +1132,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             // at most two GEMM workgroups per CU, so every CU keeps room for
             // a panel workgroup of the caller's stream
             gemm_sharing<T>(aux, trail, trail - narrow, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12 + narrow, lda,
-                            T(1), A12 + (int64_t)nbw * lda + narrow, lda, aux_gemm_reserve_lds<T>());
+                            T(1), A12 + (int64_t)nbw * lda + narrow, lda, aux_gemm_lds_footprint<T>());
         }
         LMX_HIP(hipEventRecord(evF, aux));
         pending = true;
