@@ -103,4 +103,10 @@ __device__ inline T row16_max(T v) {
     return v;
 }
 
+// Raises this wave's instruction-issue priority (s_setprio 3; waves start
+// at 0) for kernels on the factorisation's critical path: when they share a
+// CU with the lookahead GEMM, the SIMD arbiter issues their instructions
+// first.
+__device__ inline void critical_path_priority() { __builtin_amdgcn_s_setprio(3); }
+
 }  // namespace lmx

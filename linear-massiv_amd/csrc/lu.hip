@@ -336,6 +336,7 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
                                                           int32_t *__restrict__ info,
                                                           PanelSync<T> *__restrict__ sy, int P,
                                                           unsigned base) {
+    critical_path_priority();
     const int w = blockIdx.x, t = threadIdx.x;
     __shared__ PanelLds<T> L;
     PanelRows<T> st;
@@ -718,6 +719,7 @@ template <class T>
 __global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, T *__restrict__ P0, int64_t lda,
                                                            const T *__restrict__ L11, T *__restrict__ ust,
                                                            int64_t ldst) {
+    critical_path_priority();
     constexpr int kb = KB;
     constexpr int ROWS = iu_rows<T>(), RG = ROWS / 4;  // rows per workgroup, per group of 64 threads
     __shared__ __attribute__((aligned(16))) T L2[ROWS][KB];
@@ -811,6 +813,7 @@ __global__ __launch_bounds__(256) void row_gather_kernel(T *__restrict__ X, int6
                                                          const int32_t *__restrict__ srcmap,
                                                          int32_t *__restrict__ perm, int32_t *__restrict__ pairs_out,
                                                          int row_off) {
+    critical_path_priority();
     constexpr int MAXM = 2 * KB;
     constexpr int PER = MAXM / 4;  // moves per thread
     __shared__ int s_dst[MAXM], s_src[MAXM], s_cnt;
@@ -873,6 +876,7 @@ constexpr size_t aux_gemm_lds_footprint() {
 template <class T>
 __global__ __launch_bounds__(256) void row_gather_multi_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
                                                                const int32_t *__restrict__ pairs, int npanels) {
+    critical_path_priority();
     constexpr int MAXM = 2 * KB;
     constexpr int PER = MAXM / 4;
     __shared__ int s_dst[MAXM], s_src[MAXM], s_cnt;
