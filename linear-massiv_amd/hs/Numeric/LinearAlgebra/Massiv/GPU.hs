@@ -41,6 +41,7 @@ module Numeric.LinearAlgebra.Massiv.GPU
   ( matMulP
   , gemm
   , lu
+  , luFloat
   , luSolveP
   , luSolve
   , cholesky
@@ -77,6 +78,8 @@ foreign import ccall safe "lmx.h lmx_dgemm"
           -> Ptr Double -> Int64 -> CDouble -> Ptr Double -> Int64 -> IO CInt
 foreign import ccall safe "lmx.h lmx_dgetrf"
   c_dgetrf :: Int64 -> Ptr Double -> Int64 -> Ptr Int32 -> IO CInt
+foreign import ccall safe "lmx.h lmx_sgetrf"
+  c_sgetrf :: Int64 -> Ptr Float -> Int64 -> Ptr Int32 -> IO CInt
 foreign import ccall safe "lmx.h lmx_dgesv"
   c_dgesv :: Int64 -> Int64 -> Ptr Double -> Int64 -> Ptr Int32 -> Ptr Double -> Int64 -> IO CInt
 foreign import ccall safe "lmx.h lmx_dpotrf"
@@ -178,6 +181,20 @@ lu (MkMatrix a) = unsafePerformIO $ do
     swaps <- map fromIntegral <$> peekArray nn pipiv
     pure (MkMatrix f, M.fromList M.Seq (swapsToPerm nn swaps))
 {-# NOINLINE lu #-}
+
+-- | lu at e = Float (the reference's lu is generic over Fractional e,
+-- Solve/LU.hs:121): the FP32 factorisation, with the same pivot rules.
+luFloat :: forall n. KnownNat n
+        => Matrix n n M.P Float -> (Matrix n n M.P Float, M.Array M.P M.Ix1 Int)
+luFloat (MkMatrix a) = unsafePerformIO $ do
+  let nn = dimVal @n
+  allocaArray (max 1 nn) $ \pipiv -> do
+    ma <- M.thawS (M.computeAs M.S (M.delay a) :: M.Array M.S M.Ix2 Float)
+    M.withPtr ma $ \pa -> () <$ (check =<< c_sgetrf (fi nn) pa (fi (max 1 nn)) pipiv)
+    f <- M.computeAs M.P . M.delay <$> M.freezeS ma
+    swaps <- map fromIntegral <$> peekArray nn pipiv
+    pure (MkMatrix f, M.fromList M.Seq (swapsToPerm nn swaps))
+{-# NOINLINE luFloat #-}
 
 swapsToPerm :: Int -> [Int] -> [Int]
 swapsToPerm n swaps = M.toList (M.computeAs M.P (go (M.fromList M.Seq [0 .. n - 1]) (zip [0 ..] swaps)))
