@@ -565,7 +565,10 @@ def main():
         cnt, ms, work = fam[dom]["launches"], fam[dom]["ms_per_step"], fam[dom]["work_per_step"]
         achieved = work / (ms * 1e-3) / 1e12
         roofline = {"bound": "mfma",
-                    "kernel": {"gemm": "gemm_mfma_kernel<double, 128, 128, 16>",
+                    # the FP64 GEMMs of these workloads take the NN kernel (no
+                    # transposes, aligned rows); the FP32 trailing GEMM of the
+                    # mixed LU shares its CUs with the panels and keeps the first
+                    "kernel": {"gemm": "gemm_nn_kernel<double, 128, 128>",
                                "sgemm": "gemm_mfma_kernel<float, 128, 128, 16>",
                                "syrk": "gemm_mfma_kernel<double> (lower tiles)",
                                "ssyrk": "gemm_mfma_kernel<float> (lower tiles)"}[dom],
