@@ -97,3 +97,17 @@ def test_device_dgemm_8192_checksum(lmx_gpu):
     ref = A[:128].cpu().numpy() @ B.cpu().numpy()
     err = np.abs(C[:128].cpu().numpy() - ref).max()
     assert err <= 2 * n * EPS * np.abs(A[:128].cpu().numpy()).sum(1).max()
+
+
+@pytest.mark.parametrize("m,n,k", [(3000, 3000, 520), (3001, 2999, 517)])
+def test_large_tile_paths_exact(lmx_gpu, m, n, k):
+    # 128 x 128 tiles: the NN kernel when rows are 16-byte aligned (first
+    # shape), the general kernel otherwise (second); small integers keep every
+    # product and partial sum exact in both precisions, beta = 1 included
+    rng = np.random.default_rng(m + n + k)
+    A = rng.integers(-8, 9, (m, k)).astype(float)
+    B = rng.integers(-8, 9, (k, n)).astype(float)
+    C0 = rng.integers(-8, 9, (m, n)).astype(float)
+    exact = A @ B - C0  # integer-valued, |entries| < 2^24: exact in float64 and float32
+    np.testing.assert_array_equal(lmx_gpu.dgemm(A, B, C0, alpha=1.0, beta=-1.0), exact)
+    np.testing.assert_array_equal(lmx_gpu.sgemm(A, B, C0, alpha=1.0, beta=-1.0).astype(float), exact)
