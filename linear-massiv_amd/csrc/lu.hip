@@ -1120,10 +1120,15 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
         T *A12 = A11 + nbw;
         T *W = Wall + (int64_t)(b & 1) * NB_OUTER * NB_OUTER;
         const int narrow = std::min(NB_OUTER, trail);
+        // L11^-1 before the wait: it reads only this outer panel's L11 (final
+        // once its sub-panels are done; the auxiliary stream never touches
+        // these columns) and writes W of this panel's parity (last read by the
+        // auxiliary stream two panels ago), so it fills the time the caller's
+        // stream would otherwise spend waiting for the previous GEMM
+        if (trail > 0) tri_inverse_build<T>(s, true, false, true, nbw, A11, lda, W);
         if (pending) LMX_HIP(hipStreamWaitEvent(s, evF, 0));
         if (trail > 0) {
             gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
-            tri_inverse_build<T>(s, true, false, true, nbw, A11, lda, W);
             tri_inverse_solve<T>(s, true, false, nbw, narrow, A11, lda, W, A12, lda);
             gemm(s, false, false, trail, narrow, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12, lda, T(1),
                  A12 + (int64_t)nbw * lda, lda);
