@@ -2,7 +2,7 @@
 // LU solve, for row-major matrices.
 //
 // Panel kernel (getf2_panel_kernel): the m x kb panel (kb <= 32) is spread
-// over P cooperative workgroups of PT = 512 threads, each owning PT*RPT
+// over P concurrent workgroups of PT = 512 threads, each owning PT*RPT
 // consecutive rows held in registers.  Per column j:
 //   1. each workgroup finds its candidate max |a_ij| over rows i >= j (ties:
 //      lowest row, NaN never chosen -- the rule of rawPivotSearch,
@@ -42,7 +42,7 @@ constexpr int RPT = 2;        // rows per thread in the panel kernel
 constexpr int PT = 512;       // threads per panel workgroup (8 waves)
 constexpr int PW = PT / 64;   // waves per panel workgroup
 constexpr int PANEL_ROWS = PT * RPT;
-constexpr int MAXP = 64;      // max cooperative workgroups (panel height <= 32768)
+constexpr int MAXP = 64;      // max panel workgroups (panel height <= 32768)
 constexpr int NB_OUTER = 256; // outer panel width (k of the trailing GEMM)
 constexpr long SPIN_LIMIT = 1L << 22;
 
@@ -971,14 +971,18 @@ void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32
     int P = (int)cdiv(m, PANEL_ROWS);
     if (P > MAXP) fail_arg("getrf: matrix too tall for the panel kernel");
     TimedRegion tr("getrf_panel", s);
-    if (P == 1) {
-        hipLaunchKernelGGL(getf2_panel_kernel<T>, dim3(1), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
-                           moves, info, sy, 1, base);
-        LMX_LAUNCH_CHECK();
-    } else {
-        void *args[] = {&m, &kb, &A, &lda, &k0, &ipiv, &srcmap, &moves, &info, &sy, &P, &base};
-        LMX_HIP(hipLaunchCooperativeKernel((const void *)getf2_panel_kernel<T>, dim3(P), dim3(PT), args, 0, s));
-    }
+    // An ordinary launch on the caller's stream.  The P <= 64 workgroups
+    // (one CU each, of 256) wait on each other, which needs them all
+    // resident at once: a workgroup that cannot start yet (its CU busy with
+    // GEMM tiles of the other stream) starts when those tiles retire, and
+    // every wait is bounded (an error, not a hang, if one never arrives).
+    // A cooperative launch adds nothing to that here, and it runs on a
+    // hardware queue of its own: each hand-over between it and the stream's
+    // other kernels cost about 11 us (about 10 ms per FP32 factorisation at
+    // n = 16384).
+    hipLaunchKernelGGL(getf2_panel_kernel<T>, dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap, moves, info,
+                       sy, P, base);
+    LMX_LAUNCH_CHECK();
     base += (unsigned)kb;  // sequence numbers stay monotonic across launches
 }
 
