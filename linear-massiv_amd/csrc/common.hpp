@@ -62,6 +62,10 @@ struct Context {
     void *staging[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t staging_bytes[4] = {0, 0, 0, 0};
     int cu_count = 0;
+    // Scratch's memory pool: it keeps freed memory for reuse (release
+    // threshold at its maximum), so an allocation after a host
+    // synchronisation does not map memory again
+    hipMemPool_t pool = nullptr;
 };
 
 // Context of the current device (initialised on first use).
@@ -71,8 +75,8 @@ hipStream_t resolve_stream(void *s);
 // under host_mu).
 void *staging(Context &c, int slot, size_t bytes);
 
-// Stream-ordered scratch: allocated with hipMallocAsync on `s`, freed on the
-// same stream when the object goes out of scope.
+// Stream-ordered scratch: allocated on `s` from the current device's pool,
+// freed on the same stream when the object goes out of scope.
 struct Scratch {
     void *p = nullptr;
     hipStream_t s = nullptr;

@@ -50,6 +50,16 @@ Context *make_context(int dev) {
     hipDeviceProp_t prop;
     LMX_HIP(hipGetDeviceProperties(&prop, dev));
     c->cu_count = prop.multiProcessorCount;
+    // The default pool returns freed memory at every synchronisation; then the
+    // first allocations after the factorisation's info read (a host
+    // synchronisation) took 3 ms to map memory again.
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    LMX_HIP(hipMemPoolCreate(&c->pool, &props));
+    uint64_t keep = UINT64_MAX;
+    LMX_HIP(hipMemPoolSetAttribute(c->pool, hipMemPoolAttrReleaseThreshold, &keep));
     return c;
 }
 }  // namespace
@@ -94,7 +104,7 @@ void *staging(Context &c, int slot, size_t bytes) {
 
 Scratch::Scratch(size_t bytes, hipStream_t st) : s(st) {
     if (bytes == 0) bytes = 256;
-    LMX_HIP(hipMallocAsync(&p, bytes, s));
+    LMX_HIP(hipMallocFromPoolAsync(&p, bytes, ctx().pool, s));
 }
 
 Scratch::~Scratch() {
@@ -161,6 +171,9 @@ int lmx_finalize(void) {
             for (int i = 0; i < 4; ++i)
                 if (c->staging[i]) LMX_HIP(hipFree(c->staging[i]));
             LMX_HIP(hipStreamDestroy(c->stream));
+            // scratch may still be in use on callers' streams until now
+            LMX_HIP(hipDeviceSynchronize());
+            if (c->pool) LMX_HIP(hipMemPoolDestroy(c->pool));
             delete c;
             g_ctx[d] = nullptr;
         }
