@@ -21,48 +21,58 @@ namespace {
 
 constexpr int DIAG = 64;
 
-// In-place lower Cholesky of an nb x nb block (nb <= 64) in LDS.
+// Lane c's value of x, to every lane (a scalar register).
+__device__ __forceinline__ float lane_value(float x, int c) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), c));
+}
+__device__ __forceinline__ double lane_value(double x, int c) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), c),
+                            __builtin_amdgcn_readlane(__double2loint(x), c));
+}
+
+// In-place lower Cholesky of an nb x nb block (nb <= 64): one wave, lane i
+// holding row i in registers.  Step j takes the diagonal and each l_cj
+// from the lane that holds it (v_readlane into a scalar register), so a
+// step has no barrier: l_jj = sqrt(a_jj), l_ij = a_ij / l_jj (a division,
+// as rawCholColumnSIMD), then a_ic -= l_ij l_cj for c > j, in ascending j
+// for every element.  Lanes also update their row's columns above the
+// diagonal; those registers are never read or stored.
 template <class T>
-__global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
-                                                         int32_t *__restrict__ info) {
-    __shared__ T L[DIAG][DIAG + 1];
-    __shared__ int s_bad;
-    const int t = threadIdx.x;
-#pragma unroll 16
-    for (int e = t; e < nb * nb; e += 256) {
-        int r = e / nb, c = e % nb;
-        L[r][c] = (c <= r) ? A[(int64_t)r * lda + c] : T(0);
+__global__ __launch_bounds__(64) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
+                                                        int32_t *__restrict__ info) {
+    __shared__ T S[DIAG][DIAG + 1];
+    const int i = threadIdx.x;
+    // coalesced loads through LDS, then lane i takes row i
+    for (int e = i; e < nb * nb; e += 64) {
+        const int r = e / nb, c = e % nb;
+        S[r][c] = (c <= r) ? A[(int64_t)r * lda + c] : T(0);
     }
-    if (t == 0) s_bad = 0;
     __syncthreads();
-    // thread t updates row i = t % 64 at columns c = t / 64 + 4 q
-    const int i = t & 63, c0 = t >> 6;
-    for (int j = 0; j < nb; ++j) {
-        const T d = L[j][j];
-        const T sjj = sqrt(d);
-        __syncthreads();  // every thread has read L[j][j]
-        if (t == 0) {
-            if (!(d > T(0)) && s_bad == 0) s_bad = col0 + j + 1;
-            L[j][j] = sjj;
-        }
-        if (t > j && t < nb) L[t][j] = L[t][j] / sjj;
-        __syncthreads();
-        // trailing update of the lower triangle: L[i][c] -= L[i][j] L[c][j], j < c <= i
-        if (i > j && i < nb) {
-            const T lij = L[i][j];
+    T a[DIAG];
 #pragma unroll
-            for (int q = 0; q < DIAG / 4; ++q) {
-                int c = c0 + 4 * q;
-                if (c > j && c <= i) L[i][c] = fma(-lij, L[c][j], L[i][c]);
-            }
-        }
-        __syncthreads();
+    for (int c = 0; c < DIAG; ++c) a[c] = (i < nb && c < nb) ? S[i][c] : T(0);
+    // All 64 steps run (compile-time indices keep a[] in registers); rows
+    // and columns from nb on start at zero, may turn to NaN, and are never
+    // stored.
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < DIAG; ++j) {
+        const T d = lane_value(a[j], j);
+        const T sjj = sqrt(d);
+        if (j < nb && !(d > T(0)) && bad == 0) bad = col0 + j + 1;
+        a[j] = i == j ? sjj : a[j] / sjj;  // rows above j: unused registers
+#pragma unroll
+        for (int c = j + 1; c < DIAG; ++c) a[c] = fma(-a[j], lane_value(a[j], c), a[c]);
     }
-    for (int e = t; e < nb * nb; e += 256) {
-        int r = e / nb, c = e % nb;
-        if (c <= r) A[(int64_t)r * lda + c] = L[r][c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < DIAG; ++c) S[i][c] = a[c];
+    __syncthreads();
+    for (int e = i; e < nb * nb; e += 64) {
+        const int r = e / nb, c = e % nb;
+        if (c <= r) A[(int64_t)r * lda + c] = S[r][c];
     }
-    if (t == 0 && s_bad != 0 && *info == 0) *info = s_bad;
+    if (i == 0 && bad != 0 && *info == 0) *info = bad;
 }
 
 template <class T>
@@ -72,7 +82,7 @@ void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, in
         T *A11 = A + k0 * lda + k0;
         if (kb <= DIAG) {
             TimedRegion tr("potrf_diag", s);
-            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)kb, A11, lda,
+            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, s, (int)kb, A11, lda,
                                (int)(col0 + k0), d_info);
             LMX_LAUNCH_CHECK();
         } else {
@@ -105,7 +115,7 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
         T *A11 = A + k0 * lda + k0;
         if (kb <= DIAG) {
             TimedRegion tr("potrf_diag", s);
-            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)kb, A11, lda, (int)k0, d_info);
+            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, s, (int)kb, A11, lda, (int)k0, d_info);
             LMX_LAUNCH_CHECK();
         } else {
             potrf_blocked<T>(s, kb, A11, lda, k0, d_info, DIAG);
