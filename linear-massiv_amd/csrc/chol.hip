@@ -21,15 +21,6 @@ namespace {
 
 constexpr int DIAG = 64;
 
-// Lane c's value of x, to every lane (a scalar register).
-__device__ __forceinline__ float lane_value(float x, int c) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), c));
-}
-__device__ __forceinline__ double lane_value(double x, int c) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), c),
-                            __builtin_amdgcn_readlane(__double2loint(x), c));
-}
-
 // In-place lower Cholesky of an nb x nb block (nb <= 64): one wave, lane i
 // holding row i in registers.  Step j takes the diagonal and each l_cj
 // from the lane that holds it (v_readlane into a scalar register), so a

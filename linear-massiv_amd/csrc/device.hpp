@@ -109,4 +109,13 @@ __device__ inline T row16_max(T v) {
 // first.
 __device__ inline void critical_path_priority() { __builtin_amdgcn_s_setprio(3); }
 
+// Lane c's value of x, to every lane (v_readlane into a scalar register).
+__device__ __forceinline__ float lane_value(float x, int c) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), c));
+}
+__device__ __forceinline__ double lane_value(double x, int c) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), c),
+                            __builtin_amdgcn_readlane(__double2loint(x), c));
+}
+
 }  // namespace lmx
