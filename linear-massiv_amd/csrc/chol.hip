@@ -13,6 +13,7 @@
 // the first time; the factorisation continues and propagates NaN, as the
 // reference's sqrt of a negative number does.
 #include "common.hpp"
+#include "device.hpp"
 #include "kernels.hpp"
 
 namespace lmx {
