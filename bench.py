@@ -469,7 +469,8 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(the path shards by problem, see DESIGN.md)", file=sys.stderr)
+              "(one GPU holds the whole problem; more GPUs would take independent solves each)",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
