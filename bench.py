@@ -567,9 +567,12 @@ def main():
         achieved = work / (ms * 1e-3) / 1e12
         roofline = {"bound": "mfma",
                     # the FP64 GEMMs of these workloads take the NN kernel (no
-                    # transposes, aligned rows); the FP32 trailing GEMM of the
-                    # mixed LU shares its CUs with the panels and keeps the first
-                    "kernel": {"gemm": "gemm_nn_kernel<double, 128, 128>",
+                    # transposes, aligned rows): 128 x 64 tiles for the plain
+                    # GEMM, 128 x 128 beside the LU panels; the FP32 trailing
+                    # GEMM of the mixed LU shares its CUs with the panels and
+                    # keeps the first kernel
+                    "kernel": {"gemm": "gemm_nn_kernel<double, 128, 64>" if wl == "gemm_fp64"
+                               else "gemm_nn_kernel<double, 128, 128>",
                                "sgemm": "gemm_mfma_kernel<float, 128, 128, 16>",
                                "syrk": "gemm_mfma_kernel<double> (lower tiles)",
                                "ssyrk": "gemm_mfma_kernel<float> (lower tiles)"}[dom],

@@ -451,7 +451,8 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
 
 template <class T>
 void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
-               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint = 0) {
+               const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint = 0,
+               bool sharing = false) {
     if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
     if (m == 0 || n == 0) return;
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
@@ -462,8 +463,17 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     int64_t big_tiles = cdiv(m, 128) * cdiv(n, 128);
     if (big_tiles >= 512) {
         TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, flops);
-        launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
-                                     C, ldc, 0, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
+        const BatchStrides nob{1, 0, 0, 0, 0, 0, 0};
+        // FP64 NN: 128 x 64 tiles (224 registers a wave against 392 at
+        // 128 x 128, so two waves share a SIMD): dgemm 8192^3 46 -> 53 TF.  A
+        // GEMM beside the LU panels keeps 128 x 128: faster, it slowed the
+        // panels more (FP64 LU at n=8192: 75.2 against 72.4 ms).
+        if (sizeof(T) == 8 && !sharing && nn_ok<T>(ta, tb, n, k, A, lda, B, ldb, nob))
+            launch_tile<T, 128, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 1,
+                                        nob, lds_footprint);
+        else
+            launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
+                                         C, ldc, 0, 1, nob, lds_footprint);
     } else {
         TimedRegion tr(sizeof(T) == 8 ? "gemm64" : "sgemm64", s, flops);
         launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta,
@@ -525,7 +535,7 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
 template <class T>
 void gemm_sharing(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, const T *B,
                   int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint) {
-    gemm_impl<T>(s, false, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, lds_footprint);
+    gemm_impl<T>(s, false, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, lds_footprint, true);
 }
 template void gemm_sharing<double>(hipStream_t, int64_t, int64_t, int64_t, double, const double *, int64_t,
                                    const double *, int64_t, double, double *, int64_t, size_t);
