@@ -38,7 +38,7 @@ namespace lmx {
 namespace {
 
 constexpr int KB = 32;        // panel kernel width
-constexpr int RPT = 2;        // rows per thread in the panel kernel
+constexpr int RPT = 1;        // rows per thread in the panel kernel
 constexpr int PT = 512;       // threads per panel workgroup (8 waves)
 constexpr int PW = PT / 64;   // waves per panel workgroup
 constexpr int PANEL_ROWS = PT * RPT;
