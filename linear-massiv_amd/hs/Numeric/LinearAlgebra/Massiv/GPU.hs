@@ -35,8 +35,9 @@
 --
 -- The ccalls are @safe@ (they run for milliseconds and block); arguments are
 -- copied into pinned storable arrays first, which is O(n^2) against the
--- O(n^3) work.  This file is not built in this repository (no GHC in the
--- build image); INTEGRATION.md gives the cabal stanza.
+-- O(n^3) work.  This file is not built here (no GHC in this image); a
+-- package that adds it lists liblmx in extra-libraries and include/ in
+-- include-dirs.
 module Numeric.LinearAlgebra.Massiv.GPU
   ( matMulP
   , gemm
