@@ -172,48 +172,74 @@ __global__ __launch_bounds__(256) void tri_diag_copy_kernel(int n, int bw, const
 
 // out[r][q] = alpha sum_{c<k} M[r][c] x[c][q] (added to out[r][q] when
 // `accumulate`), r < rows, q < nrhs <= NR, k <= IBW.  One wave per row, rpw
-// rows per wave; x is staged in LDS column by column (consecutive lanes read
-// consecutive banks) and a lane's 16 row elements are all loaded before the
-// products, so a wave has its whole row in flight.
+// rows per wave (rpw * NR <= 64); x is staged in LDS column by column
+// (consecutive lanes read consecutive banks).  The loads are pipelined one
+// row ahead in two register buffers used in turn (no copy between them): the
+// first row is in flight while x is staged, and a row's products wait only
+// for that row while the next one loads.  Lane rr * NR + q keeps row rr's
+// result for column q and the wave stores its rows once at the end, so no
+// store inside the loop orders the loads behind it.  The prefetch row index
+// is clamped to rows - 1, so every load is in bounds.  FULL (k == IBW) drops
+// the column mask, so the loads carry no branches.
+template <class T, bool FULL>
+__device__ __forceinline__ void gemv_load_row(const T *__restrict__ M, int64_t ldm, int64_t r, int k, int lane,
+                                              T (&mv)[IBW / 64]) {
+    const T *mr = M + r * ldm;
+#pragma unroll
+    for (int i = 0; i < IBW / 64; ++i) {
+        const int c = lane + 64 * i;
+        mv[i] = FULL || c < k ? mr[c] : T(0);
+    }
+}
+
 template <class T, int NR>
+__device__ __forceinline__ void gemv_finish_row(const T (&mv)[IBW / 64], const T (&xs)[NR][IBW], int lane, int rr,
+                                                T &res) {
+    T acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = T(0);
+#pragma unroll
+    for (int i = 0; i < IBW / 64; ++i)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] = fma(mv[i], xs[q][lane + 64 * i], acc[q]);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const T v = wave_sum(acc[q]);
+        res = lane == rr * NR + q ? v : res;
+    }
+}
+
+template <class T, int NR, bool FULL>
 __global__ __launch_bounds__(256) void gemv_rows_kernel(int rows, int k, int nrhs, int rpw, T alpha,
                                                         const T *__restrict__ M, int64_t ldm,
                                                         const T *__restrict__ x, int64_t ldx, int accumulate,
                                                         T *__restrict__ out, int64_t ldo) {
     __shared__ T xs[NR][IBW];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t rbeg = ((int64_t)blockIdx.x * 4 + wave) * rpw, last = rows - 1;
+    // this lane's output element: row rbeg + lane / NR, column lane % NR
+    const int64_t orow = rbeg + lane / NR;
+    const int oq = lane % NR;
+    const bool mine = lane / NR < rpw && orow < rows && oq < nrhs;
+    T old = T(0), res = T(0);
+    if (accumulate && mine) old = out[orow * ldo + oq];
+    T b0[IBW / 64], b1[IBW / 64];
+    gemv_load_row<T, FULL>(M, ldm, rbeg < last ? rbeg : last, k, lane, b0);
     for (int e = t; e < NR * IBW; e += 256) {
         const int c = e / NR, q = e % NR;
         xs[q][c] = (c < k && q < nrhs) ? x[(int64_t)c * ldx + q] : T(0);
     }
     __syncthreads();
-    const int64_t rbeg = ((int64_t)blockIdx.x * 4 + wave) * rpw;
-    for (int rr = 0; rr < rpw; ++rr) {
+    for (int rr = 0; rr < rpw; rr += 2) {
         const int64_t r = rbeg + rr;
         if (r >= rows) break;  // wave-uniform
-        const T *mr = M + r * ldm;
-        T mv[IBW / 64];
-#pragma unroll
-        for (int i = 0; i < IBW / 64; ++i) {
-            const int c = lane + 64 * i;
-            mv[i] = c < k ? mr[c] : T(0);
-        }
-        T acc[NR];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] = T(0);
-#pragma unroll
-        for (int i = 0; i < IBW / 64; ++i)
-#pragma unroll
-            for (int q = 0; q < NR; ++q) acc[q] = fma(mv[i], xs[q][lane + 64 * i], acc[q]);
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            const T v = wave_sum(acc[q]);
-            if (lane == q && q < nrhs) {
-                T *o = out + r * ldo + q;
-                *o = accumulate ? fma(alpha, v, *o) : alpha * v;
-            }
-        }
+        gemv_load_row<T, FULL>(M, ldm, r + 1 < last ? r + 1 : last, k, lane, b1);
+        gemv_finish_row<T, NR>(b0, xs, lane, rr, res);
+        if (rr + 1 >= rpw || r + 1 >= rows) break;  // wave-uniform
+        gemv_load_row<T, FULL>(M, ldm, r + 2 < last ? r + 2 : last, k, lane, b0);
+        gemv_finish_row<T, NR>(b1, xs, lane, rr + 1, res);
     }
+    if (mine) out[orow * ldo + oq] = accumulate ? fma(alpha, res, old) : alpha * res;
 }
 
 template <class T>
@@ -264,11 +290,12 @@ void apply_block(hipStream_t s, int64_t rows, int64_t k, int64_t nrhs, T alpha, 
         gemm(s, mtrans, false, rows, nrhs, k, alpha, M, ldm, x, ldx, accumulate ? T(1) : T(0), out, ldo);
         return;
     }
+    // rows per wave: at most 8, so rpw * NR <= 64 result lanes for NR <= 8
     const int rpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, rows / 4096));
     dim3 g((unsigned)cdiv(rows, 4 * rpw)), blk(256);
-#define LMX_GEMV(NR_)                                                                                           \
-    hipLaunchKernelGGL((gemv_rows_kernel<T, NR_>), g, blk, 0, s, (int)rows, (int)k, (int)nrhs, rpw, alpha, M, ldm, \
-                       x, ldx, accumulate ? 1 : 0, out, ldo)
+#define LMX_GEMV(NR_)                                                                                          \
+    hipLaunchKernelGGL((k == IBW ? gemv_rows_kernel<T, NR_, true> : gemv_rows_kernel<T, NR_, false>), g, blk, 0, \
+                       s, (int)rows, (int)k, (int)nrhs, rpw, alpha, M, ldm, x, ldx, accumulate ? 1 : 0, out, ldo)
     if (nrhs == 1)
         LMX_GEMV(1);
     else if (nrhs == 2)

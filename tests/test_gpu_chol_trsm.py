@@ -108,6 +108,20 @@ def test_trsm_vs_scipy(lmx_gpu, uplo, trans, diag, n, nrhs):
     np.testing.assert_allclose(X, want, rtol=1e-11, atol=1e-11)
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,nrhs", [(10000, 1), (13500, 3)])
+def test_trsm_many_rows_per_wave(lmx_gpu, uplo, n, nrhs):
+    # the row updates after each 1024-row block run 2 (n = 10000) and 3
+    # (n = 13500) rows per wave; 13500 also ends in a ragged 188-row block
+    rng = np.random.default_rng(n + nrhs)
+    T = rng.uniform(-1, 1, (n, n)) / n
+    T[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    B = rng.uniform(-1, 1, (n, nrhs))
+    X = lmx_gpu.trsm(T, B, uplo, "N", "N")
+    want = sl.solve_triangular(T, B, lower=(uplo == "L"))
+    np.testing.assert_allclose(X, want, rtol=1e-11, atol=1e-11)
+
+
 def test_reference_substitutions_vs_oracle(lmx_gpu, orc):
     rng = np.random.default_rng(5)
     n = 90

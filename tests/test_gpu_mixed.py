@@ -38,6 +38,18 @@ def test_gesv_mixed_f32(lmx_gpu, orc, kappa, n, nrhs):
     assert err <= 10 * orc.condition_tolerance(kappa, n)
 
 
+def test_gesv_mixed_f32_many_rows_per_wave(lmx_gpu, orc):
+    # n = 9300: the FP32 triangular solves of the refinement run 2 rows per
+    # wave in their first row update and end in a ragged 84-row block
+    n = 9300
+    rng = np.random.default_rng(93)
+    A = rng.uniform(-1, 1, (n, n))
+    B = rng.uniform(-1, 1, (n, 1))
+    X, ri = lmx_gpu.gesv_mixed(A, B, lmx_gpu.PREC_F32)
+    assert ri.converged == 1 and ri.fell_back == 0
+    assert orc.scaled_residual_linear(A, X, B) <= 10 * n * EPS
+
+
 @pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
 @pytest.mark.parametrize("n,nrhs", [(300, 1), (1000, 4)])
 def test_posv_mixed_f32(lmx_gpu, orc, kappa, n, nrhs):
