@@ -155,41 +155,68 @@ __global__ __launch_bounds__(256) void colmax_few_kernel(const double *__restric
     if (t < ncols) atomic_max_nonneg(&out[t], fmax(fmax(part[0][t], part[1][t]), fmax(part[2][t], part[3][t])));
 }
 
-// R = B - A X in FP64 for nrhs <= NR: one wave per row of A (coalesced
-// row reads, many rows in flight), wave-shuffle reduction.
+// R = B - A X in FP64 for nrhs <= NR: each wave takes RESID_RPW rows of A
+// and walks them together, so one load of x[k] serves all of them and the
+// wave keeps RESID_RPW row loads in flight per step (coalesced row reads);
+// wave-shuffle reduction.  A wave's rows past n repeat row n - 1 (in bounds)
+// and are not stored.
+constexpr int RESID_RPW = 4;
+
 template <int NR>
 __global__ __launch_bounds__(256) void resid_kernel(int64_t n, int nrhs, const double *__restrict__ A, int64_t lda,
                                                     const double *__restrict__ X, int64_t ldx,
                                                     const double *__restrict__ B, int64_t ldb,
                                                     double *__restrict__ R, int64_t ldr) {
+    constexpr int RW = RESID_RPW;
     const int lane = threadIdx.x & 63;
-    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    double acc[NR];
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
+    if (i0 >= n) return;  // wave-uniform
+    const double *a[RW];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = 0.0;
-    const double *a = A + i * lda;
-#pragma unroll 4
-    for (int64_t k = lane; k < n; k += 64) {
-        double av = a[k];
+    for (int r = 0; r < RW; ++r) a[r] = A + (i0 + r < n ? i0 + r : n - 1) * lda;
+    double acc[RW][NR];
 #pragma unroll
-        for (int c = 0; c < NR; ++c)
-            if (c < nrhs) acc[c] = fma(av, X[k * ldx + c], acc[c]);
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[r][c] = 0.0;
+    const int64_t nfull = n / 256 * 256;
+    for (int64_t k0 = 0; k0 < nfull; k0 += 256) {
+        double av[4][RW], xv[4][NR];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t k = k0 + 64 * u + lane;
+#pragma unroll
+            for (int r = 0; r < RW; ++r) av[u][r] = a[r][k];
+#pragma unroll
+            for (int c = 0; c < NR; ++c) xv[u][c] = X[k * ldx + (c < nrhs ? c : 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+                for (int c = 0; c < NR; ++c) acc[r][c] = fma(av[u][r], xv[u][c], acc[r][c]);
     }
+    for (int64_t k = nfull + lane; k < n; k += 64) {
 #pragma unroll
-    for (int c = 0; c < NR; ++c) {
-        double v = acc[c];
+        for (int r = 0; r < RW; ++r) {
+            const double av = a[r][k];
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-        acc[c] = v;
+            for (int c = 0; c < NR; ++c) acc[r][c] = fma(av, X[k * ldx + (c < nrhs ? c : 0)], acc[r][c]);
+        }
     }
-    if (lane < nrhs && lane < NR) {
-        double v = 0.0;
+    double out = 0.0;
 #pragma unroll
-        for (int c = 0; c < NR; ++c)
-            if (c == lane) v = acc[c];
-        R[i * ldr + lane] = B[i * ldb + lane] - v;
-    }
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            double v = acc[r][c];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+            out = lane == r * NR + c ? v : out;
+        }
+    const int r = lane / NR, c = lane % NR;
+    if (r < RW && i0 + r < n && c < nrhs) R[(i0 + r) * ldr + c] = B[(i0 + r) * ldb + c] - out;
 }
 
 void colmax(hipStream_t s, const double *X, int64_t ldx, int64_t n, int ncols, double *out) {
@@ -207,8 +234,11 @@ void colmax(hipStream_t s, const double *X, int64_t ldx, int64_t n, int ncols, d
 void residual(hipStream_t s, int64_t n, int64_t k, const double *A, int64_t lda, const double *X, int64_t ldx,
               const double *B, int64_t ldb, double *R, int64_t ldr) {
     if (k <= 4) {
-        hipLaunchKernelGGL(resid_kernel<4>, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, s, n, (int)k, A, lda, X, ldx,
-                           B, ldb, R, ldr);
+        const dim3 g((unsigned)cdiv(n, 4 * RESID_RPW));
+        if (k == 1)
+            hipLaunchKernelGGL(resid_kernel<1>, g, dim3(256), 0, s, n, 1, A, lda, X, ldx, B, ldb, R, ldr);
+        else
+            hipLaunchKernelGGL(resid_kernel<4>, g, dim3(256), 0, s, n, (int)k, A, lda, X, ldx, B, ldb, R, ldr);
         LMX_LAUNCH_CHECK();
         return;
     }
