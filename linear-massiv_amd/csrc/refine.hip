@@ -156,18 +156,53 @@ __global__ __launch_bounds__(256) void colmax_few_kernel(const double *__restric
 }
 
 // R = B - A X in FP64 for nrhs <= NR: each wave takes RESID_RPW rows of A
-// and walks them together, so one load of x[k] serves all of them and the
-// wave keeps RESID_RPW row loads in flight per step (coalesced row reads);
-// wave-shuffle reduction.  A wave's rows past n repeat row n - 1 (in bounds)
-// and are not stored.
+// and walks them together, so one load of x[k] serves all of them (coalesced
+// row reads).  The columns go in pairs of chunks of RESID_CH * 64, pipelined
+// one chunk ahead in two register buffers used in turn: a chunk's products
+// wait only for that chunk while the next one loads.  The loop has no exit
+// between the two halves of a pair (an exit there lets the compiler move the
+// prefetch down to its use); columns past the last full pair go through a
+// plain tail loop.  The prefetch chunk is clamped to the last full one, so
+// every load is in bounds.  A wave's rows past n repeat
+// row n - 1 (in bounds) and are not stored.  Wave-shuffle reduction.
 constexpr int RESID_RPW = 4;
+constexpr int RESID_CH = 2;
+
+template <int NR>
+struct ResidChunk {
+    double a[RESID_CH][RESID_RPW], x[RESID_CH][NR];
+};
+
+template <int NR>
+__device__ __forceinline__ void resid_load(ResidChunk<NR> &b, const double *const (&a)[RESID_RPW],
+                                           const double *__restrict__ X, int64_t ldx, int nrhs, int64_t k0,
+                                           int lane) {
+#pragma unroll
+    for (int u = 0; u < RESID_CH; ++u) {
+        const int64_t k = k0 + 64 * u + lane;
+#pragma unroll
+        for (int r = 0; r < RESID_RPW; ++r) b.a[u][r] = a[r][k];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) b.x[u][c] = X[k * ldx + (c < nrhs ? c : 0)];
+    }
+}
+
+template <int NR>
+__device__ __forceinline__ void resid_fma(const ResidChunk<NR> &b, double (&acc)[RESID_RPW][NR]) {
+#pragma unroll
+    for (int u = 0; u < RESID_CH; ++u)
+#pragma unroll
+        for (int r = 0; r < RESID_RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[r][c] = fma(b.a[u][r], b.x[u][c], acc[r][c]);
+}
 
 template <int NR>
 __global__ __launch_bounds__(256) void resid_kernel(int64_t n, int nrhs, const double *__restrict__ A, int64_t lda,
                                                     const double *__restrict__ X, int64_t ldx,
                                                     const double *__restrict__ B, int64_t ldb,
                                                     double *__restrict__ R, int64_t ldr) {
-    constexpr int RW = RESID_RPW;
+    constexpr int RW = RESID_RPW, CW = RESID_CH * 64;
     const int lane = threadIdx.x & 63;
     const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
     if (i0 >= n) return;  // wave-uniform
@@ -179,23 +214,18 @@ __global__ __launch_bounds__(256) void resid_kernel(int64_t n, int nrhs, const d
     for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int c = 0; c < NR; ++c) acc[r][c] = 0.0;
-    const int64_t nfull = n / 256 * 256;
-    for (int64_t k0 = 0; k0 < nfull; k0 += 256) {
-        double av[4][RW], xv[4][NR];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t k = k0 + 64 * u + lane;
-#pragma unroll
-            for (int r = 0; r < RW; ++r) av[u][r] = a[r][k];
-#pragma unroll
-            for (int c = 0; c < NR; ++c) xv[u][c] = X[k * ldx + (c < nrhs ? c : 0)];
+    const int64_t nfull = n / (2 * CW) * (2 * CW), klast = nfull - CW;
+    if (nfull > 0) {
+        ResidChunk<NR> b0, b1;
+        resid_load<NR>(b0, a, X, ldx, nrhs, 0, lane);
+        for (int64_t k0 = 0; k0 < nfull; k0 += 2 * CW) {
+            resid_load<NR>(b1, a, X, ldx, nrhs, k0 + CW, lane);
+            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
+            resid_fma<NR>(b0, acc);
+            resid_load<NR>(b0, a, X, ldx, nrhs, k0 + 2 * CW < klast ? k0 + 2 * CW : klast, lane);
+            __builtin_amdgcn_sched_barrier(0);
+            resid_fma<NR>(b1, acc);
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int r = 0; r < RW; ++r)
-#pragma unroll
-                for (int c = 0; c < NR; ++c) acc[r][c] = fma(av[u][r], xv[u][c], acc[r][c]);
     }
     for (int64_t k = nfull + lane; k < n; k += 64) {
 #pragma unroll
