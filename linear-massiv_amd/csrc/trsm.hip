@@ -230,15 +230,19 @@ __global__ __launch_bounds__(256) void gemv_rows_kernel(int rows, int k, int nrh
         xs[q][c] = (c < k && q < nrhs) ? x[(int64_t)c * ldx + q] : T(0);
     }
     __syncthreads();
-    for (int rr = 0; rr < rpw; rr += 2) {
+    // pairs of rows with no exit between their halves (an exit there lets
+    // the compiler move the prefetch down to its use), then an odd last row
+    int rr = 0;
+    for (; rr + 1 < rpw && rbeg + rr + 1 < rows; rr += 2) {
         const int64_t r = rbeg + rr;
-        if (r >= rows) break;  // wave-uniform
-        gemv_load_row<T, FULL>(M, ldm, r + 1 < last ? r + 1 : last, k, lane, b1);
+        gemv_load_row<T, FULL>(M, ldm, r + 1, k, lane, b1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
         gemv_finish_row<T, NR>(b0, xs, lane, rr, res);
-        if (rr + 1 >= rpw || r + 1 >= rows) break;  // wave-uniform
         gemv_load_row<T, FULL>(M, ldm, r + 2 < last ? r + 2 : last, k, lane, b0);
+        __builtin_amdgcn_sched_barrier(0);
         gemv_finish_row<T, NR>(b1, xs, lane, rr + 1, res);
     }
+    if (rr < rpw && rbeg + rr < rows) gemv_finish_row<T, NR>(b0, xs, lane, rr, res);
     if (mine) out[orow * ldo + oq] = accumulate ? fma(alpha, res, old) : alpha * res;
 }
 
