@@ -31,9 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "linear-massiv_amd"))
 
 EPS = 2.220446049250313e-16
-PEAK = {"gemm": 78.6, "sgemm": 157.3, "syrk": 78.6, "ssyrk": 157.3}  # TFLOP/s, MI355X dense MFMA
-KERNEL_FAMILIES = ["gemm", "sgemm", "gemm64", "sgemm64", "syrk", "ssyrk", "getrf_panel", "trsm", "laswp",
-                   "potrf_diag"]
+# TFLOP/s, MI355X dense MFMA (bgemm / bgemm3: the BF16 tiles, 2.5 PFLOP/s)
+PEAK = {"gemm": 78.6, "sgemm": 157.3, "syrk": 78.6, "ssyrk": 157.3, "bgemm": 2500.0, "bgemm3": 2500.0}
+KERNEL_FAMILIES = ["gemm", "sgemm", "gemm64", "sgemm64", "syrk", "ssyrk", "bgemm", "bgemm3", "getrf_panel", "trsm",
+                   "laswp", "potrf_diag"]
 # HBM bytes per launch of a workload's dominant kernel, from the rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (workload, n) -> file
 TRAFFIC_PROFILES = {("lu_mixed", 16384): "sgemm_lu_mixed_n16384_traffic.md"}
@@ -58,8 +59,10 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="lu_mixed",
-                   choices=["lu_mixed", "chol_mixed", "lu_fp64", "chol_fp64", "gemm_fp64", "batch_mixed",
-                            "batch_fp64", "serve"])
+                   choices=["lu_mixed", "chol_mixed", "lu_fp64", "chol_fp64", "gemm_fp64", "gemm_bf16",
+                            "gemm_bf16x3", "batch_mixed", "batch_fp64", "serve"])
+    p.add_argument("--factor", default="f32", choices=["f32", "bf16"],
+                   help="mixed workloads: the factor precision (bf16: BF16 tiles with split operands, GMRES-IR)")
     p.add_argument("--requests", type=int, default=400, help="serve: requests per rate point")
     p.add_argument("--count", type=int, default=20000, help="problems per family (batched workloads)")
     p.add_argument("--autotune", action="store_true", help="batched workloads: autotune the kernel variant")
@@ -147,16 +150,27 @@ def cpu_baseline(workload, n, nrhs, batch=None):
         return {"value": len(shapes) / t_full, "unit": "solves/s", "cores": 1, "kind": "port",
                 "sample": f"oracle luSolveP / choleskySolveP on {len(idx)} random problems of the batch, one call "
                           f"per RHS, 1 thread ({t_used:.1f} s), scaled by flops to the {len(shapes)} problems"}
-    if workload == "gemm_fp64":
-        ns = 2048
+    if workload.startswith("gemm"):
+        # matMulPPar (BLAS/Level3.hs:213-294, gridDims 2D partition) on every
+        # host core the box gives this job, and matMulP on one for reference
+        cores = max(1, min(16, os.cpu_count() or 1))
+        ns = 4096
         A = rng.uniform(-1, 1, (ns, ns))
         B = rng.uniform(-1, 1, (ns, ns))
-        t = time.perf_counter()
-        oracle.matMulP(A, B)
-        dt = time.perf_counter() - t
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            oracle.matMulPPar(A, B, cores)
+            times.append(time.perf_counter() - t)
+        dt = sorted(times)[1]
         flops = 2.0 * ns ** 3
-        return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-                "sample": f"oracle matMulP {ns}x{ns}x{ns}, 1 run, 1 thread ({dt:.1f} s)"}
+        t = time.perf_counter()
+        oracle.matMulP(A[:2048, :2048].copy(), B[:2048, :2048].copy())
+        dt1 = time.perf_counter() - t
+        return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+                "one_thread_tflops": 2.0 * 2048 ** 3 / dt1 / 1e12,
+                "sample": f"oracle matMulPPar {ns}x{ns}x{ns} on {cores} threads, median of 3 runs "
+                          f"({dt:.2f} s); matMulP 2048^3 on 1 thread ({dt1:.1f} s) beside it"}
     ns = 4096 if n >= 4096 else n
     A = rng.uniform(-1, 1, (ns, ns))
     b = rng.uniform(-1, 1, ns)
@@ -164,16 +178,21 @@ def cpu_baseline(workload, n, nrhs, batch=None):
     if spd:
         G = rng.standard_normal((ns, ns))
         A = G @ G.T / ns + np.eye(ns)
-    t = time.perf_counter()
-    (oracle.choleskySolveP if spd else oracle.luSolveP)(A, b)
-    dt = time.perf_counter() - t
+    reps = 3
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        (oracle.choleskySolveP if spd else oracle.luSolveP)(A, b)
+        times.append(time.perf_counter() - t)
+    dt = sorted(times)[reps // 2]
     # The reference API takes one right-hand side per call and re-factors on
     # every call, so nrhs right-hand sides cost nrhs calls.
     calls = max(nrhs, 1)
     t_full = dt * (n / ns) ** 3 * calls
     name = "choleskySolveP" if spd else "luSolveP"
     return {"value": 1.0 / t_full, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle {name} n={ns}, 1 RHS, 1 run, 1 thread ({dt:.1f} s), scaled by (n/{ns})^3 to n={n}"
+            "sample": f"oracle {name} n={ns}, 1 RHS, median of {reps} runs, 1 thread ({dt:.1f} s), "
+                      f"scaled by (n/{ns})^3 to n={n}"
                       + (f" x {calls} calls (one per RHS, as the reference API)" if calls > 1 else "")}
 
 
@@ -483,16 +502,22 @@ def main():
     if wl == "serve":
         return bench_serve(a, np, torch, lmx)
     defaults = {"lu_mixed": (16384, 1), "chol_mixed": (16384, 1), "lu_fp64": (8192, 16),
-                "chol_fp64": (8192, 16), "gemm_fp64": (8192, 0)}
+                "chol_fp64": (8192, 16), "gemm_fp64": (8192, 0), "gemm_bf16": (8192, 0), "gemm_bf16x3": (8192, 0)}
+    gemm_wl = wl.startswith("gemm")
+    prec = lmx.PREC_BF16 if a.factor == "bf16" else lmx.PREC_F32
     n = a.n or defaults[wl][0]
     nrhs = a.nrhs if a.nrhs is not None else defaults[wl][1]
     g = torch.Generator(device="cuda").manual_seed(a.seed)
     spd = wl.startswith("chol")
     t0 = time.perf_counter()
-    if wl == "gemm_fp64":
+    if gemm_wl:
         A = uniform(torch, (n, n), g)
         B = uniform(torch, (n, n), g)
-        C = torch.empty_like(A)
+        if wl == "gemm_bf16":
+            A, B = A.to(torch.bfloat16), B.to(torch.bfloat16)
+        elif wl == "gemm_bf16x3":
+            A, B = A.float(), B.float()
+        C = torch.empty(n, n, dtype=torch.float64 if wl == "gemm_fp64" else torch.float32, device="cuda")
     elif wl in ("lu_mixed", "chol_mixed"):
         A = randsvd(torch, n, a.kappa, g, spd)
         B = uniform(torch, (n, nrhs), g)
@@ -504,7 +529,7 @@ def main():
     log(f"[bench] {wl} n={n} nrhs={nrhs}: inputs ready in {time.perf_counter() - t0:.1f} s")
 
     # FP64 path work buffers
-    if wl != "gemm_fp64":
+    if not gemm_wl:
         LU = torch.empty_like(A)
         XB = torch.empty_like(B)
         ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
@@ -515,8 +540,12 @@ def main():
     def step():
         if wl == "gemm_fp64":
             lmx.dgemm_dev(A, B, C)
+        elif wl == "gemm_bf16":
+            lmx.gemm_bf16_dev(A, B, C)
+        elif wl == "gemm_bf16x3":
+            lmx.sgemm_bf16x3_dev(A, B, C)
         elif wl in ("lu_mixed", "chol_mixed"):
-            ri = (lmx.posv_mixed_dev if spd else lmx.gesv_mixed_dev)(A, B, X, lmx.PREC_F32)
+            ri = (lmx.posv_mixed_dev if spd else lmx.gesv_mixed_dev)(A, B, X, prec)
             last["ri"] = ri
         else:
             LU.copy_(A)
@@ -559,7 +588,7 @@ def main():
     lmx.kernel_timing_reset()
 
     # dominant MFMA kernel family and its roofline
-    mf = [f for f in ("gemm", "sgemm", "syrk", "ssyrk") if f in fam]
+    mf = [f for f in ("gemm", "sgemm", "syrk", "ssyrk", "bgemm", "bgemm3") if f in fam]
     dom = max(mf, key=lambda f: fam[f]["ms_per_step"]) if mf else None
     roofline = None
     if dom:
@@ -575,7 +604,9 @@ def main():
                                else "gemm_nn_kernel<double, 128, 128>",
                                "sgemm": "gemm_mfma_kernel<float, 128, 128, 16>",
                                "syrk": "gemm_mfma_kernel<double> (lower tiles)",
-                               "ssyrk": "gemm_mfma_kernel<float> (lower tiles)"}[dom],
+                               "ssyrk": "gemm_mfma_kernel<float> (lower tiles)",
+                               "bgemm": "gemm_bf16_kernel<bf16, 128, 128>",
+                               "bgemm3": "gemm_bf16_kernel<float, split, 128, 128> (3 BF16 MFMAs a product)"}[dom],
                     "achieved": round(achieved, 3), "peak": PEAK[dom], "unit": "TFLOP/s",
                     "frac": round(achieved / PEAK[dom], 4), "traffic": profiled_traffic(wl, n),
                     "launches_per_step": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2)}
@@ -587,10 +618,12 @@ def main():
     p50 = float(np.percentile(lat_sorted, 50))
     p99 = float(np.percentile(lat_sorted, 99))
 
-    if wl == "gemm_fp64":
+    if gemm_wl:
         flops = 2.0 * n ** 3
-        result.update({"value": round(flops / (ms_per_step * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "dtype": "f64",
-                       "config": {"workload": f"GEMM C = A B, {n}x{n}x{n}, FP64 MFMA, A, B iid U[-1,1]", "n": n}})
+        kind = {"gemm_fp64": ("f64", "FP64 MFMA"), "gemm_bf16": ("bf16", "BF16 MFMA, FP32 accumulation"),
+                "gemm_bf16x3": ("f32 (bf16x3)", "FP32 operands split in two BF16 parts, 3 BF16 MFMAs a product")}[wl]
+        result.update({"value": round(flops / (ms_per_step * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "dtype": kind[0],
+                       "config": {"workload": f"GEMM C = A B, {n}x{n}x{n}, {kind[1]}, A, B iid U[-1,1]", "n": n}})
     else:
         if spd:
             flops = n ** 3 / 3.0 + 2.0 * n * n * nrhs
@@ -605,11 +638,15 @@ def main():
                        "backward_error_bound_10neps": 10 * n * EPS})
         if wl.endswith("mixed"):
             ri = last["ri"]
-            result["dtype"] = "f32+f64"
-            result["refine"] = {"iterations": ri.iterations, "converged": ri.converged, "fell_back": ri.fell_back}
-            result["config"] = {"workload": f"{name} FP32-factor + FP64 refinement, n={n}, {nrhs} RHS, "
+            bf = a.factor == "bf16"
+            result["dtype"] = "bf16x3+f64" if bf else "f32+f64"
+            result["refine"] = {"iterations": ri.iterations, "converged": ri.converged, "fell_back": ri.fell_back,
+                                "method": "gmres-ir" if bf else "ir"}
+            fac = "BF16-factor (BF16 tiles, split operands) + FP64 GMRES-IR" if bf else \
+                "FP32-factor + FP64 refinement"
+            result["config"] = {"workload": f"{name} {fac}, n={n}, {nrhs} RHS, "
                                             f"A = U diag(sigma) V^T (Haar, kappa={a.kappa:g})",
-                                "n": n, "nrhs": nrhs, "kappa": a.kappa, "factor": "fp32"}
+                                "n": n, "nrhs": nrhs, "kappa": a.kappa, "factor": a.factor}
             if not a.no_fp64_compare:
                 k2 = max(2, a.steps // 2)
                 for _ in range(1):

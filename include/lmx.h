@@ -28,7 +28,10 @@
  *     compute and copy back.  `_dev` functions take device pointers and a
  *     hipStream_t (passed as void*, NULL = HIP's default stream) and are
  *     stream-ordered; their `info` goes to a device int32 so no host
- *     synchronisation is needed.
+ *     synchronisation is needed.  A device `info` of -1 means the LU
+ *     panel's cross-workgroup exchange did not complete within its bounded
+ *     wait (its workgroups could not all be resident at once): the factors
+ *     are then not valid.  Host functions report it as LMX_ERR_INTERNAL.
  *   - Every function is thread-safe.
  */
 #ifndef LMX_H
@@ -88,12 +91,38 @@ int lmx_sgemm(char transa, char transb, int64_t m, int64_t n, int64_t k,
               float alpha, const float *A, int64_t lda, const float *B,
               int64_t ldb, float beta, float *C, int64_t ldc);
 
+/* C (FP32) = alpha op(A) op(B) + beta C on the BF16 matrix cores
+ * (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; A and B hold BF16 bit
+ * patterns (the upper half of an IEEE single).  matMul at a BF16 element
+ * type: the reference multiplies any Num e through the generic path
+ * (src/.../BLAS/Level3.hs:150-181); this is the BF16 tile the survey's GEMM
+ * row asks for beside FP64 and FP32. */
+int lmx_gemm_bf16(char transa, char transb, int64_t m, int64_t n, int64_t k,
+                  float alpha, const uint16_t *A, int64_t lda,
+                  const uint16_t *B, int64_t ldb, float beta, float *C,
+                  int64_t ldc);
+int lmx_gemm_bf16_dev(void *stream, char transa, char transb, int64_t m,
+                      int64_t n, int64_t k, float alpha, const uint16_t *dA,
+                      int64_t lda, const uint16_t *dB, int64_t ldb,
+                      float beta, float *dC, int64_t ldc);
+/* FP32 operands on the BF16 matrix cores: each element is split as
+ * x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) and every product is
+ * hi*hi + hi*lo + lo*hi, accumulated in FP32 (relative error about 2^-16
+ * per product).  This is the trailing-update GEMM of the BF16 factor of
+ * the mixed-precision solvers (LMX_PREC_BF16). */
+int lmx_sgemm_bf16x3_dev(void *stream, char transa, char transb, int64_t m,
+                         int64_t n, int64_t k, float alpha, const float *dA,
+                         int64_t lda, const float *dB, int64_t ldb,
+                         float beta, float *dC, int64_t ldc);
+
 /* ---- LU ----------------------------------------------------------------- */
 
 /* In-place LU with partial pivoting of the n x n matrix A: unit L below the
  * diagonal, U on and above it; ipiv[k] = pivot row of step k (n entries).
- * Replaces the factorisation of luSolveP (panelLUFactor,
- * src/.../Solve/LU.hs:276-284, :311-357) and of lu (:121-167). */
+ * The pivot of each column is its first row of largest magnitude, the rule
+ * of rawPivotSearch (src/.../Internal/Kernel.hs:1541-1556).  Replaces the
+ * factorisation of luSolveP (panelLUFactor, src/.../Solve/LU.hs:276-284,
+ * :311-357). */
 int lmx_dgetrf(int64_t n, double *A, int64_t lda, int32_t *ipiv);
 int lmx_dgetrf_dev(void *stream, int64_t n, double *dA, int64_t lda,
                    int32_t *d_ipiv, int32_t *d_info);
@@ -103,6 +132,19 @@ int lmx_dgetrf_dev(void *stream, int64_t n, double *dA, int64_t lda,
 int lmx_sgetrf(int64_t n, float *A, int64_t lda, int32_t *ipiv);
 int lmx_sgetrf_dev(void *stream, int64_t n, float *dA, int64_t lda,
                    int32_t *d_ipiv, int32_t *d_info);
+/* The generic lu (src/.../Solve/LU.hs:121-167): A is overwritten by the
+ * packed factors and perm (n entries) receives lu's permutation vector: row
+ * i of PA is row perm[i] of A.  Pivots follow lu's maximumBy (:128-135): the
+ * LAST row of largest |a_ik| wins a tie, a NaN is chosen only where it sits
+ * on the diagonal, and a zero pivot leaves its column unscaled (:151-152).
+ * The return value is the 1-based column of the first zero pivot, or 0.
+ * lmx_slu is lu at e = Float. */
+int lmx_dlu(int64_t n, double *A, int64_t lda, int32_t *perm);
+int lmx_dlu_dev(void *stream, int64_t n, double *dA, int64_t lda,
+                int32_t *d_perm, int32_t *d_info);
+int lmx_slu(int64_t n, float *A, int64_t lda, int32_t *perm);
+int lmx_slu_dev(void *stream, int64_t n, float *dA, int64_t lda,
+                int32_t *d_perm, int32_t *d_info);
 /* Solve A X = B with the factors of lmx_dgetrf; B (n x nrhs) is
  * overwritten by X.  Replaces the solve phase of luSolveP
  * (src/.../Solve/LU.hs:294-303: applyPivotsForward,
@@ -158,6 +200,75 @@ int lmx_dtrsm_dev(void *stream, char uplo, char trans, char diag, int64_t n,
                   int64_t nrhs, const double *dT, int64_t ldt, double *dB,
                   int64_t ldb);
 
+/* ---- level-2 utilities, norms, SYRK, determinant, inverse iteration ---- */
+
+/* y = alpha A x + beta y for an m x n row-major A (x and y unit stride).
+ * Replaces gemv (src/.../BLAS/Level2.hs:92-100), matvec / matvecP
+ * (:123-144, alpha = 1, beta = 0; rawGemv, Internal/Kernel.hs:119-134).
+ * beta = 0 does not read y (BLAS convention; the reference's gemv forms
+ * beta * y_i, so a NaN in y reaches its result only there). */
+int lmx_dgemv(int64_t m, int64_t n, double alpha, const double *A, int64_t lda,
+              const double *x, double beta, double *y);
+int lmx_dgemv_dev(void *stream, int64_t m, int64_t n, double alpha,
+                  const double *dA, int64_t lda, const double *dx, double beta,
+                  double *dy);
+/* A matrix norm of the m x n A into *out: which = '1' (max column sum,
+ * norm1), 'I' (max row sum, normInf), 'F' (Frobenius, normFrob) or 'M'
+ * (max |a_ij|).  Replaces src/.../Norms.hs norm1 (:124-130), normInf
+ * (:142-148) and normFrob (:110-112); a vector is an n x 1 matrix (vnorm1
+ * = '1', :72-74; vnorm2 = 'F', :83-85; vnormInf = 'I', :94-96).  A NaN
+ * entry gives +Inf for '1', 'I' and 'M'. */
+int lmx_dnorm(char which, int64_t m, int64_t n, const double *A, int64_t lda,
+              double *out);
+int lmx_dnorm_dev(void *stream, char which, int64_t m, int64_t n,
+                  const double *dA, int64_t lda, double *d_out);
+/* R = B - A X in FP64 (A n x n, X, B, R n x nrhs); the residual the
+ * refinement loop and test/Test/Residuals.hs:69-78 form. */
+int lmx_dresidual_dev(void *stream, int64_t n, int64_t nrhs, const double *dA,
+                      int64_t lda, const double *dX, int64_t ldx,
+                      const double *dB, int64_t ldb, double *dR, int64_t ldr);
+/* eta[c] = ||A x_c - b_c||_inf / (||A||_inf ||x_c||_inf + ||b_c||_inf + n eps)
+ * for each of the nrhs columns: scaledResidualLinear
+ * (test/Test/Residuals.hs:69-78), the backward error the refinement stops
+ * on.  nrhs <= 16 is formed without storing the residual. */
+int lmx_dscaled_residual(int64_t n, int64_t nrhs, const double *A, int64_t lda,
+                         const double *X, int64_t ldx, const double *B,
+                         int64_t ldb, double *eta);
+int lmx_dscaled_residual_dev(void *stream, int64_t n, int64_t nrhs,
+                             const double *dA, int64_t lda, const double *dX,
+                             int64_t ldx, const double *dB, int64_t ldb,
+                             double *d_eta);
+/* C = alpha op(A) op(A)^T + beta C, the whole symmetric n x n C (its lower
+ * triangle computed on the FP64 MFMA tiles, then mirrored).  trans 'N':
+ * op(A) = A (n x k); 'T': op(A) = A^T with A stored k x n, so alpha = 1,
+ * beta = 0 is matMulAtAP (src/.../BLAS/Level3.hs:353-373,
+ * rawSyrkLowerKernel Internal/Kernel.hs:812-1058). */
+int lmx_dsyrk(char trans, int64_t n, int64_t k, double alpha, const double *A,
+              int64_t lda, double beta, double *C, int64_t ldc);
+int lmx_dsyrk_dev(void *stream, char trans, int64_t n, int64_t k,
+                  double alpha, const double *dA, int64_t lda, double beta,
+                  double *dC, int64_t ldc);
+/* det A (A not modified): lu's factors (its tie rule), the product of U's
+ * diagonal formed left to right, and the sign of the permutation.  Replaces
+ * det (src/.../Solve/LU.hs:534-558). */
+int lmx_ddet(int64_t n, const double *A, int64_t lda, double *det);
+int lmx_ddet_dev(void *stream, int64_t n, const double *dA, int64_t lda,
+                 double *d_det);
+/* Inverse iteration (src/.../Eigen/Power.hs:113-135): with q = q0 and
+ * lambda_prev = 0, repeat z = (A - mu I)^{-1} q, q' = z / ||z||_2,
+ * lambda = q'^T A q' until |lambda - lambda_prev| < tol (returning lambda,
+ * q') or max_iter steps (returning the last lambda and q).  A - mu I is
+ * factored once with lu's pivots; each step is a solve with the resident
+ * factor, a norm, a GEMV and a dot product on the device, and lambda is read
+ * back for the test.  *iterations receives the number of steps taken. */
+int lmx_dinverse_iteration(int64_t n, const double *A, int64_t lda, double mu,
+                           const double *q0, int32_t max_iter, double tol,
+                           double *lambda, double *q, int32_t *iterations);
+int lmx_dinverse_iteration_dev(void *stream, int64_t n, const double *dA,
+                               int64_t lda, double mu, const double *dq0,
+                               int32_t max_iter, double tol, double *lambda,
+                               double *dq, int32_t *iterations);
+
 /* ---- mixed precision with FP64 iterative refinement -------------------- */
 
 /* Outcome of a refined solve. */
@@ -190,6 +301,42 @@ int lmx_dposv_mixed_dev(void *stream, int precision, int64_t n, int64_t nrhs,
                         const double *dA, int64_t lda, const double *dB,
                         int64_t ldb, double *dX, int64_t ldx,
                         lmx_refine_info *info);
+
+/* ---- resident factors ---------------------------------------------------- */
+
+/* A factor kept in HBM for repeated solves.  The reference's lu and
+ * cholesky (src/.../Solve/LU.hs:121, Solve/Cholesky.hs:114) return factors
+ * its solvers never reuse (luSolveP and choleskySolveP refactor on every
+ * call, LU.hs:265-304, Cholesky.hs:241-273); a handle factors once.
+ *   kind: LMX_FACTOR_LU (partial pivoting) or LMX_FACTOR_CHOLESKY (lower).
+ *   precision: LMX_PREC_F64 (the FP64 factor; solves are the permutation and
+ *     two triangular solves through the kept diagonal-block inverses) or
+ *     LMX_PREC_F32 / LMX_PREC_BF16 (the low-precision factor plus an FP64
+ *     copy of A; every solve is refined to FP64 accuracy as
+ *     lmx_dgesv_mixed does, with the same fallback).
+ * create returns the factorisation's info (> 0: singular / not positive
+ * definite; the handle is still created) or an error; A is not modified.
+ * Solves only read the handle: several may run at once (on different
+ * streams); release waits for the device and must not overlap them. */
+typedef struct lmx_factor_s *lmx_factor;
+#define LMX_FACTOR_LU 0
+#define LMX_FACTOR_CHOLESKY 1
+int lmx_factor_create(lmx_factor *h, int kind, int precision, int64_t n,
+                      const double *A, int64_t lda);
+int lmx_factor_create_dev(lmx_factor *h, void *stream, int kind,
+                          int precision, int64_t n, const double *dA,
+                          int64_t lda);
+/* X (n x nrhs) = A^{-1} B for the handle's A.  Returns 0, the factor's info
+ * (FP64 handles), LMX_ERR_NOT_CONVERGED, or an error; `info` (may be NULL)
+ * receives the refinement outcome (iterations 0 for FP64 handles). */
+int lmx_factor_solve(lmx_factor h, int64_t nrhs, const double *B, int64_t ldb,
+                     double *X, int64_t ldx, lmx_refine_info *info);
+int lmx_factor_solve_dev(lmx_factor h, void *stream, int64_t nrhs,
+                         const double *dB, int64_t ldb, double *dX,
+                         int64_t ldx, lmx_refine_info *info);
+int lmx_factor_query(lmx_factor h, int32_t *info, int64_t *n, int32_t *kind,
+                     int32_t *precision, int64_t *bytes);
+int lmx_factor_release(lmx_factor h);
 
 /* ---- batched solves of many small systems ------------------------------- */
 

@@ -67,8 +67,26 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(int nb, T *__restrict__ 
     if (i == 0 && bad != 0 && *info == 0) *info = bad;
 }
 
+// C -= A B^T (lower tiles only with `lower`): FP64 / FP32 MFMA, or the BF16
+// tiles with split operands for the BF16 factor.
 template <class T>
-void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, int32_t *d_info, int64_t nb) {
+void chol_update(hipStream_t s, bool bf16, bool lower, int64_t m, int64_t n, int64_t k, const T *A, int64_t lda,
+                 const T *B, T *C) {
+    if constexpr (sizeof(T) == 4) {
+        if (bf16) {
+            gemm_bf16x3(s, false, true, m, n, k, -1.0f, A, lda, B, lda, 1.0f, C, lda, lower);
+            return;
+        }
+    }
+    if (lower)
+        syrk_lower(s, m, k, T(-1), A, lda, T(1), C, lda);
+    else
+        gemm(s, false, true, m, n, k, T(-1), A, lda, B, lda, T(1), C, lda);
+}
+
+template <class T>
+void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, int32_t *d_info, int64_t nb,
+                   bool bf16) {
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         const int64_t kb = std::min(nb, n - k0);
         T *A11 = A + k0 * lda + k0;
@@ -78,13 +96,13 @@ void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, in
                                (int)(col0 + k0), d_info);
             LMX_LAUNCH_CHECK();
         } else {
-            potrf_blocked<T>(s, kb, A11, lda, col0 + k0, d_info, DIAG);
+            potrf_blocked<T>(s, kb, A11, lda, col0 + k0, d_info, DIAG, bf16);
         }
         const int64_t rest = n - k0 - kb;
         if (rest > 0) {
             T *A21 = A11 + kb * lda;
             trsm_right<T>(s, true, true, false, rest, kb, A11, lda, A21, lda);
-            syrk_lower(s, rest, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
+            chol_update<T>(s, bf16, true, rest, rest, kb, A21, lda, A21, A21 + kb);
         }
     }
 }
@@ -99,7 +117,7 @@ void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, in
 // While they overlap the streams write disjoint columns (b + 1's against
 // those beyond it), and the auxiliary stream only reads block column b.
 template <class T>
-void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, int64_t nb) {
+void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, int64_t nb, bool bf16) {
     AuxStream la;
     bool pending = false;
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
@@ -110,7 +128,7 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
             hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, s, (int)kb, A11, lda, (int)k0, d_info);
             LMX_LAUNCH_CHECK();
         } else {
-            potrf_blocked<T>(s, kb, A11, lda, k0, d_info, DIAG);
+            potrf_blocked<T>(s, kb, A11, lda, k0, d_info, DIAG, bf16);
         }
         const int64_t rest = n - k0 - kb;
         T *A21 = A11 + kb * lda;
@@ -118,16 +136,16 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
         if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
         const int64_t narrow = std::min(nb, rest);
         if (rest > 0) {
-            syrk_lower(s, narrow, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
+            chol_update<T>(s, bf16, true, narrow, narrow, kb, A21, lda, A21, A21 + kb);
             if (rest > narrow)
-                gemm(s, false, true, rest - narrow, narrow, kb, T(-1), A21 + narrow * lda, lda, A21, lda, T(1),
-                     A21 + narrow * lda + kb, lda);
+                chol_update<T>(s, bf16, false, rest - narrow, narrow, kb, A21 + narrow * lda, lda, A21,
+                               A21 + narrow * lda + kb);
         }
         LMX_HIP(hipEventRecord(la.ready, s));
         LMX_HIP(hipStreamWaitEvent(la.aux, la.ready, 0));
         if (rest > narrow)
-            syrk_lower(la.aux, rest - narrow, kb, T(-1), A21 + narrow * lda, lda, T(1), A21 + narrow * lda + kb + narrow,
-                       lda);
+            chol_update<T>(la.aux, bf16, true, rest - narrow, rest - narrow, kb, A21 + narrow * lda, lda,
+                           A21 + narrow * lda, A21 + narrow * lda + kb + narrow);
         LMX_HIP(hipEventRecord(la.done, la.aux));
         pending = true;
     }
@@ -137,13 +155,14 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
 }  // namespace
 
 template <class T>
-void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info) {
+void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, bool bf16) {
     if (n < 0 || lda < std::max<int64_t>(1, n)) fail_arg("potrf: bad dimension or stride");
     if (n > INT32_MAX / 2) fail_arg("potrf: n too large");
     clear_info(s, d_info);
     if (n == 0) return;
     TimedRegion trall("potrf", s, (double)n * (double)n * (double)n / 3.0);
-    potrf_lookahead<T>(s, n, A, lda, d_info, 128);
+    if (bf16 && sizeof(T) != 4) fail_arg("potrf: BF16 updates are for the FP32-stored factor");
+    potrf_lookahead<T>(s, n, A, lda, d_info, 128, bf16);
 }
 
 template <class T>
@@ -159,8 +178,8 @@ void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B
     trsm_left<T>(s, true, true, false, n, nrhs, L, lda, B, ldb);
 }
 
-template void potrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *);
-template void potrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *);
+template void potrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, bool);
+template void potrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, bool);
 template void potrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, double *, int64_t,
                             const double *, const double *);
 template void potrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, float *, int64_t, const float *,

@@ -75,6 +75,15 @@ __device__ inline int wave_min(int v) {
     v = min(v, dpp_i<0x143, 0xC>(v));
     return __builtin_amdgcn_readlane(v, 63);
 }
+__device__ inline int wave_max_int(int v) {
+    v = max(v, dpp_i<0xB1, 0xF>(v));
+    v = max(v, dpp_i<0x4E, 0xF>(v));
+    v = max(v, dpp_i<0x124, 0xF>(v));
+    v = max(v, dpp_i<0x128, 0xF>(v));
+    v = max(v, dpp_i<0x142, 0xA>(v));
+    v = max(v, dpp_i<0x143, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
 template <class T>
 __device__ inline T wave_sum(T v) {
     v += dpp_f<0xB1, 0xF>(v);

@@ -393,6 +393,269 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
         }
 }
 
+// ---------------------------------------------------------------- BF16 tiles
+//
+// C (FP32) = alpha op(A) op(B) + beta C on v_mfma_f32_16x16x32_bf16 (FP32
+// accumulation).  The operands are either BF16 in memory (S = uint16_t, the
+// bit pattern) or FP32 rounded to BF16 on the way into LDS (S = float).
+// With SPLIT (FP32 operands only) each element x becomes hi = bf16(x) and
+// lo = bf16(x - hi), and every block product is hi*hi + hi*lo + lo*hi:
+// three BF16 MFMAs recover about 16 significant bits of each product (the
+// dropped lo*lo term is below 2^-16 of it).
+//   Tiles are BM x BN x 32 (one MFMA k-chunk per LDS stage), 256 threads in a
+//   2 x 2 grid of waves, double-buffered LDS with one barrier per stage.  In
+//   LDS both operands are k-contiguous (As[i][k], Bs[j][k]) so that a lane's
+//   MFMA operand (8 consecutive k of one row: A[i = l%16][k = 8(l/16)+0..7],
+//   B[k = 8(l/16)+0..7][j = l%16]) is one 128-bit read; the row stride of 40
+//   BF16 (80 bytes) keeps those reads off each other's banks.
+//   An operand whose k runs along its stored rows (A as stored, B transposed)
+//   is read in VE-element vectors along k and written as VE-element runs; an
+//   operand whose rows run along memory (B as stored, A transposed) is read
+//   as two VE-element vectors at k and k + 1 and written as VE packed BF16
+//   pairs, the lanes of a wave on consecutive k pairs so the writes fall on
+//   distinct banks.  VE = 4 needs every row 16-byte (FP32) or 8-byte (BF16)
+//   aligned and the vector dimension a multiple of 4; VE = 1 takes any shape.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline unsigned short bf16_bits(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
+__device__ inline float bf16_value(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <class S, bool SPLIT>
+__device__ inline void bf16_parts(S x, unsigned short &hi, unsigned short &lo) {
+    if constexpr (sizeof(S) == 2) {
+        hi = x;
+        lo = 0;
+    } else {
+        hi = bf16_bits(x);
+        lo = SPLIT ? bf16_bits(x - bf16_value(hi)) : (unsigned short)0;
+    }
+}
+
+template <class S, int VE>
+struct alignas(sizeof(S) * VE) SVec {
+    S v[VE];
+};
+
+constexpr int BF_BK = 32;
+constexpr int BF_SK = 40;
+
+// One operand tile of R rows (of op(X)) x BF_BK k, KC = k contiguous in memory.
+template <class S, bool KC, int R, int VE, bool SPLIT>
+struct BfTile {
+    static constexpr int TASKS = KC ? R * BF_BK / VE / 256 : R * BF_BK / (2 * VE) / 256;
+    static_assert(TASKS >= 1, "tile too small for the vector width");
+    SVec<S, VE> r[TASKS][KC ? 1 : 2];
+
+    // op(X)[row][kk] is X[row * ldx + kk] (KC) or X[kk * ldx + row]
+    __device__ void load(const S *__restrict__ X, int64_t ldx, int r0, int rows, int k0, int kdim) {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int it = 0; it < TASKS; ++it) {
+            const int e = it * 256 + tid;
+            if (KC) {
+                const int row = e / (BF_BK / VE), kq = e % (BF_BK / VE);
+                const int gr = r0 + row, gk = k0 + kq * VE;
+                if (gr < rows && gk < kdim)
+                    r[it][0] = *reinterpret_cast<const SVec<S, VE> *>(X + (int64_t)gr * ldx + gk);
+                else
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) r[it][0].v[q] = S(0);
+            } else {
+                const int kp = e % (BF_BK / 2), rq = e / (BF_BK / 2);
+                const int gr = r0 + rq * VE;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int gk = k0 + 2 * kp + h;
+                    if (gr < rows && gk < kdim)
+                        r[it][h] = *reinterpret_cast<const SVec<S, VE> *>(X + (int64_t)gk * ldx + gr);
+                    else
+#pragma unroll
+                        for (int q = 0; q < VE; ++q) r[it][h].v[q] = S(0);
+                }
+            }
+        }
+    }
+    __device__ void store(unsigned short (*hi)[BF_SK], unsigned short (*lo)[BF_SK]) const {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int it = 0; it < TASKS; ++it) {
+            const int e = it * 256 + tid;
+            if (KC) {
+                const int row = e / (BF_BK / VE), kq = e % (BF_BK / VE);
+                unsigned short h[VE], l[VE];
+#pragma unroll
+                for (int q = 0; q < VE; ++q) bf16_parts<S, SPLIT>(r[it][0].v[q], h[q], l[q]);
+#pragma unroll
+                for (int q = 0; q < VE; ++q) {
+                    hi[row][kq * VE + q] = h[q];
+                    if constexpr (SPLIT) lo[row][kq * VE + q] = l[q];
+                }
+            } else {
+                const int kp = e % (BF_BK / 2), rq = e / (BF_BK / 2);
+#pragma unroll
+                for (int q = 0; q < VE; ++q) {
+                    unsigned short h0, l0, h1, l1;
+                    bf16_parts<S, SPLIT>(r[it][0].v[q], h0, l0);
+                    bf16_parts<S, SPLIT>(r[it][1].v[q], h1, l1);
+                    *reinterpret_cast<unsigned *>(&hi[rq * VE + q][2 * kp]) = (unsigned)h0 | ((unsigned)h1 << 16);
+                    if constexpr (SPLIT)
+                        *reinterpret_cast<unsigned *>(&lo[rq * VE + q][2 * kp]) = (unsigned)l0 | ((unsigned)l1 << 16);
+                }
+            }
+        }
+    }
+};
+
+template <class S, bool TA, bool TB, bool SPLIT, int BM, int BN, int VE>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int m, int n, int k, float alpha, const S *__restrict__ A,
+                                                        int64_t lda, const S *__restrict__ B, int64_t ldb,
+                                                        float beta, float *__restrict__ C, int64_t ldc, int tiles_m,
+                                                        int tiles_n, int lower) {
+    constexpr int NP = SPLIT ? 2 : 1;
+    constexpr int WM = BM / 2, WN = BN / 2, MT = WM / 16, NT = WN / 16;
+    __shared__ __attribute__((aligned(16))) unsigned short As[2][NP][BM][BF_SK];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NP][BN][BF_SK];
+    int tm, tn;
+    tile_of(tiles_m, tiles_n, lower != 0, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f4 acc[MT][NT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) acc[a][b] = f4{0, 0, 0, 0};
+
+    BfTile<S, !TA, BM, VE, SPLIT> ta;
+    BfTile<S, TB, BN, VE, SPLIT> tb;
+    const int KT = (k + BF_BK - 1) / BF_BK;
+    if (KT > 0) {
+        ta.load(A, lda, m0, m, 0, k);
+        tb.load(B, ldb, n0, n, 0, k);
+        ta.store(As[0][0], As[0][NP - 1]);
+        tb.store(Bs[0][0], Bs[0][NP - 1]);
+        __syncthreads();
+    }
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < KT;
+        if (more) {
+            ta.load(A, lda, m0, m, (kt + 1) * BF_BK, k);
+            tb.load(B, ldb, n0, n, (kt + 1) * BF_BK, k);
+        }
+        bf16x8 fa[NP][MT], fb[NP][NT];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int a = 0; a < MT; ++a)
+                fa[p][a] = *reinterpret_cast<const bf16x8 *>(&As[cur][p][wm0 + a * 16 + lr][8 * lk]);
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+                fb[p][b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][p][wn0 + b * 16 + lr][8 * lk]);
+        }
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                if constexpr (SPLIT) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1][a], fb[0][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][a], fb[1][b], acc[a][b], 0, 0, 0);
+                }
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+            }
+        if (more) {
+            ta.store(As[cur ^ 1][0], As[cur ^ 1][NP - 1]);
+            tb.store(Bs[cur ^ 1][0], Bs[cur ^ 1][NP - 1]);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+            const int gj = n0 + wn0 + b * 16 + lr;
+            if (gj >= n) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm0 + a * 16 + acc_row<float>(lk, r);
+                if (gi < m && (!lower || gj <= gi)) {
+                    float *c = C + (int64_t)gi * ldc + gj;
+                    float v = alpha * acc[a][b][r];
+                    if (beta != 0.0f) v += beta * *c;
+                    *c = v;
+                }
+            }
+        }
+}
+
+template <class S, bool SPLIT, int BM, int BN, int VE>
+void launch_bf16_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, float alpha, const S *A, int64_t lda,
+                      const S *B, int64_t ldb, float beta, float *C, int64_t ldc, int lower, size_t lds_footprint) {
+    const int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
+    const dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n);
+    const size_t st = 2 * (size_t)(SPLIT ? 2 : 1) * (BM + BN) * BF_SK * sizeof(unsigned short);
+    const size_t reserve = lds_footprint > st ? lds_footprint - st : 0;
+#define LMX_BF16_CASE(TA_, TB_)                                                                                   \
+    hipLaunchKernelGGL((gemm_bf16_kernel<S, TA_, TB_, SPLIT, BM, BN, VE>), grid, dim3(256), reserve, s, m, n, k, \
+                       alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, lower)
+    if (!ta && !tb)
+        LMX_BF16_CASE(false, false);
+    else if (!ta && tb)
+        LMX_BF16_CASE(false, true);
+    else if (ta && !tb)
+        LMX_BF16_CASE(true, false);
+    else
+        LMX_BF16_CASE(true, true);
+#undef LMX_BF16_CASE
+    LMX_LAUNCH_CHECK();
+}
+
+// VE = 4 applies when every vector is wholly inside or outside the operand
+// and aligned: rows aligned to 4 elements (16 bytes FP32, 8 bytes BF16) and
+// the vector dimension (k for a k-contiguous operand, its rows otherwise) a
+// multiple of 4.
+template <class S>
+bool bf16_vec_ok(bool ta, bool tb, int64_t m, int64_t n, int64_t k, const S *A, int64_t lda, const S *B,
+                 int64_t ldb) {
+    auto al = [](const void *p) { return ((uintptr_t)p & (4 * sizeof(S) - 1)) == 0; };
+    const bool a_ok = ta ? (m % 4 == 0) : (k % 4 == 0);
+    const bool b_ok = tb ? (k % 4 == 0) : (n % 4 == 0);
+    return al(A) && al(B) && lda % 4 == 0 && ldb % 4 == 0 && a_ok && b_ok;
+}
+
+template <class S, bool SPLIT>
+void gemm_bf16_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const S *A,
+                    int64_t lda, const S *B, int64_t ldb, float beta, float *C, int64_t ldc, int lower,
+                    size_t lds_footprint) {
+    if (m < 0 || n < 0 || k < 0) fail_arg("gemm_bf16: negative dimension");
+    if (m == 0 || n == 0) return;
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm_bf16: dimension too large");
+    if (lower && m != n) fail_arg("gemm_bf16: lower tiles need a square C");
+    const double flops = (lower ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
+    TimedRegion tr(SPLIT ? "bgemm3" : "bgemm", s, flops);
+    const bool vec = bf16_vec_ok<S>(ta, tb, m, n, k, A, lda, B, ldb);
+    const int64_t big = lower ? cdiv(m, 128) * (cdiv(m, 128) + 1) / 2 : cdiv(m, 128) * cdiv(n, 128);
+    const int M = (int)m, N = (int)n, K = (int)k;
+    if (big >= 512) {
+        if (vec)
+            launch_bf16_tile<S, SPLIT, 128, 128, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower,
+                                                    lds_footprint);
+        else
+            launch_bf16_tile<S, SPLIT, 128, 128, 1>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower,
+                                                    lds_footprint);
+    } else {
+        if (vec)
+            launch_bf16_tile<S, SPLIT, 64, 64, 4>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower,
+                                                  lds_footprint);
+        else
+            launch_bf16_tile<S, SPLIT, 64, 64, 1>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower,
+                                                  lds_footprint);
+    }
+}
+
 // LDS of one workgroup of the two kernels (static arrays)
 template <class T, int BM, int BN, int BK>
 constexpr size_t gemm_lds() {
@@ -493,26 +756,50 @@ void gemm_batched_impl(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, 
                                nb1 * nb2, bs);
 }
 
+// Lower tiles of C = alpha op(A) op(A)^T + beta C: op(A) = A (A A^T, the
+// Cholesky update) or, with `trans`, A^T (A^T A of matMulAtAP).
 template <class T>
 void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
-               int64_t ldc) {
+               int64_t ldc, bool trans = false) {
     if (n <= 0) return;
     if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
     TimedRegion tr(sizeof(T) == 8 ? "syrk" : "ssyrk", s, (double)n * (double)(n + 1) * (double)k);
     int64_t big_tiles = cdiv(n, 128) * (cdiv(n, 128) + 1) / 2;
     if (big_tiles >= 512)
-        launch_tile<T, 128, 128, 16>(s, false, true, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
+        launch_tile<T, 128, 128, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
                                      ldc, 1);
     else
-        launch_tile<T, 64, 64, 16>(s, false, true, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
+        launch_tile<T, 64, 64, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
                                    ldc, 1);
 }
 
 }  // namespace
 
+void gemm_bf16x3(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                 int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, bool lower,
+                 size_t lds_footprint) {
+    gemm_bf16_impl<float, true>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, lower ? 1 : 0,
+                                lds_footprint);
+}
+
+void gemm_bf16(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *A,
+               int64_t lda, const uint16_t *B, int64_t ldb, float beta, float *C, int64_t ldc) {
+    gemm_bf16_impl<uint16_t, false>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0);
+}
+
+void gemm_bf16_rounded(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                       int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc) {
+    gemm_bf16_impl<float, false>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0);
+}
+
 void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda, double beta,
                 double *C, int64_t ldc) {
     syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+void syrk_lower_op(hipStream_t s, bool trans, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                   double beta, double *C, int64_t ldc) {
+    syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc, trans);
 }
 
 void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, float beta,
@@ -635,6 +922,53 @@ int lmx_sgemm_dev(void *stream, char transa, char transb, int64_t m, int64_t n, 
         bool ta = is_trans(transa), tb = is_trans(transb);
         check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
         gemm(resolve_stream(stream), ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        return LMX_OK;
+    });
+}
+
+int lmx_gemm_bf16_dev(void *stream, char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                      const uint16_t *dA, int64_t lda, const uint16_t *dB, int64_t ldb, float beta, float *dC,
+                      int64_t ldc) {
+    return guarded([&] {
+        bool ta = is_trans(transa), tb = is_trans(transb);
+        check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
+        gemm_bf16(resolve_stream(stream), ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        return LMX_OK;
+    });
+}
+
+int lmx_sgemm_bf16x3_dev(void *stream, char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha,
+                         const float *dA, int64_t lda, const float *dB, int64_t ldb, float beta, float *dC,
+                         int64_t ldc) {
+    return guarded([&] {
+        bool ta = is_trans(transa), tb = is_trans(transb);
+        check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
+        gemm_bf16x3(resolve_stream(stream), ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, false, 0);
+        return LMX_OK;
+    });
+}
+
+int lmx_gemm_bf16(char transa, char transb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *A,
+                  int64_t lda, const uint16_t *B, int64_t ldb, float beta, float *C, int64_t ldc) {
+    return guarded([&] {
+        bool ta = is_trans(transa), tb = is_trans(transb);
+        check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
+        if (m == 0 || n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        const int64_t arows = ta ? k : m, brows = tb ? n : k, acols = ta ? m : k, bcols = tb ? k : n;
+        auto ext = [](int64_t rows, int64_t cols, int64_t ld, size_t es) -> size_t {
+            return (rows > 0 && cols > 0) ? (size_t)((rows - 1) * ld + cols) * es : 0;
+        };
+        const size_t ab = ext(arows, acols, lda, 2), bb = ext(brows, bcols, ldb, 2), cb = ext(m, n, ldc, 4);
+        auto *dA = (uint16_t *)staging(c, 0, ab), *dB = (uint16_t *)staging(c, 1, bb);
+        auto *dC = (float *)staging(c, 2, cb);
+        if (ab) LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        if (bb) LMX_HIP(hipMemcpyAsync(dB, B, bb, hipMemcpyHostToDevice, c.stream));
+        if (beta != 0.0f || ldc > n) LMX_HIP(hipMemcpyAsync(dC, C, cb, hipMemcpyHostToDevice, c.stream));
+        gemm_bf16(c.stream, ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        LMX_HIP(hipMemcpyAsync(C, dC, cb, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipStreamSynchronize(c.stream));
         return LMX_OK;
     });
 }

@@ -20,6 +20,21 @@ void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, floa
           const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
           int64_t ldc);
 
+// C = alpha op(A) op(B) + beta C for FP32 C on BF16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation):
+//   gemm_bf16x3: FP32 operands split into hi = bf16(x) and lo = bf16(x - hi),
+//     products hi*hi + hi*lo + lo*hi (about 16 significant bits); with
+//     `lower`, C is square and only its lower triangle's tiles are computed;
+//   gemm_bf16: BF16 operands (bit patterns) as stored;
+//   gemm_bf16_rounded: FP32 operands rounded once to BF16.
+void gemm_bf16x3(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                 int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc, bool lower = false,
+                 size_t lds_footprint = 0);
+void gemm_bf16(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const uint16_t *A,
+               int64_t lda, const uint16_t *B, int64_t ldb, float beta, float *C, int64_t ldc);
+void gemm_bf16_rounded(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                       int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc);
+
 // C = alpha A B + beta C with each workgroup holding lds_footprint bytes of
 // LDS in all (its tiles plus an unused reserve): fewer GEMM workgroups share
 // a CU, which leaves room for the workgroups of a kernel running beside it
@@ -42,6 +57,51 @@ void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double 
                 double beta, double *C, int64_t ldc);
 void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, float beta,
                 float *C, int64_t ldc);
+
+// Lower triangle of C = alpha op(A) op(A)^T + beta C: op(A) = A (n x k) or,
+// with `trans`, A^T (A stored k x n, so C = A^T A).
+void syrk_lower_op(hipStream_t s, bool trans, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                   double beta, double *C, int64_t ldc);
+// The same, then the lower triangle mirrored into the upper one (a full
+// symmetric C, as matMulAtAP returns).
+void syrk_full(hipStream_t s, bool trans, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+               double beta, double *C, int64_t ldc);
+
+// ---- level-2 work (blas2.hip)
+// R = B - A X in FP64 (A n x n, X and B n x k).
+void residual(hipStream_t s, int64_t n, int64_t k, const double *A, int64_t lda, const double *X, int64_t ldx,
+              const double *B, int64_t ldb, double *R, int64_t ldr);
+// colmax_out[c] = max(colmax_out[c], max_i |(B - A X)[i][c]|), A m x n,
+// without storing the residual.
+void residual_colmax(hipStream_t s, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *X,
+                     int64_t ldx, const double *B, int64_t ldb, double *colmax_out);
+// Y = A X in FP64 (A m x n, X n x k, Y m x k): the GEMV kernel for k <= 4,
+// the MFMA GEMM otherwise.
+void matmul_block(hipStream_t s, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *X,
+                  int64_t ldx, double *Y, int64_t ldy);
+// y = alpha A x + beta y (A m x n; beta = 0 does not read y).
+void gemv(hipStream_t s, int64_t m, int64_t n, double alpha, const double *A, int64_t lda, const double *x,
+          double beta, double *y);
+// out[c] = max(out[c], max_i |X[i][c]|) (NaN counts as +inf).
+void colmax(hipStream_t s, const double *X, int64_t ldx, int64_t n, int ncols, double *out);
+// *out = max(*out, ||A||_inf) for A m x n.
+void norm_inf_acc(hipStream_t s, const double *A, int64_t lda, int64_t m, int64_t n, double *out);
+// *d_out = the norm `which` of A: '1' (max column sum), 'I' (max row sum),
+// 'F' (Frobenius), 'M' (max |a_ij|).
+void matrix_norm(hipStream_t s, char which, int64_t m, int64_t n, const double *A, int64_t lda, double *d_out);
+// d_eta[c] = scaledResidualLinear of column c; acc holds 1 + 3k doubles of
+// device memory and R n x k doubles (read only when k > SCALED_RESIDUAL_FUSED_MAX).
+constexpr int64_t SCALED_RESIDUAL_FUSED_MAX = 16;
+void scaled_residual(hipStream_t s, int64_t n, int64_t k, const double *A, int64_t lda, const double *X, int64_t ldx,
+                     const double *B, int64_t ldb, double *d_eta, double *acc, double *R);
+// *d_det = det A from lu's factors (A is not modified).
+void determinant(hipStream_t s, int64_t n, const double *A, int64_t lda, double *d_det);
+void d2f(hipStream_t s, const double *in, int64_t ldi, float *out, int64_t ldo, int64_t rows, int64_t cols);
+// X += D (or X = D with `assign`), D in FP32.
+void f2d_acc(hipStream_t s, const float *D, int64_t ldd, double *X, int64_t ldx, int64_t rows, int64_t cols,
+             bool assign);
+void copy_d(hipStream_t s, const double *in, int64_t ldi, double *out, int64_t ldo, int64_t rows, int64_t cols);
+void zero_d(hipStream_t s, double *p, int64_t n);
 
 // Triangular solve with many right-hand sides, in place on B (n x nrhs):
 // op(T) X = B, T read from the `lower` or upper triangle, `unit` diagonal
@@ -73,9 +133,13 @@ void getrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t
 // d_perm[i] of A) when d_perm is not null.  `tournament` chooses each
 // panel's pivots by a tournament over the panel's row blocks (CALU) instead
 // of column-by-column partial pivoting: one exchange per panel.
+// `tie_last` breaks ties between equal pivot magnitudes towards the highest
+// row (the generic lu's maximumBy) instead of the lowest (rawPivotSearch).
+// `bf16` (FP32 factors only) runs the trailing updates on the BF16 tiles with
+// split operands (gemm_bf16x3).
 template <class T>
 void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
-                int32_t *d_info, bool tournament = false);
+                int32_t *d_info, bool tournament = false, bool tie_last = false, bool bf16 = false);
 // Solve with getrf's factors and its permutation vector; WL / WU, when not
 // null, are tri_inverse_build's inverses of L (unit lower) and U.
 template <class T>
@@ -86,9 +150,10 @@ template <class T>
 void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda,
            const int32_t *d_ipiv, T *B, int64_t ldb);
 
-// Lower Cholesky, in place; writes *d_info.
+// Lower Cholesky, in place; writes *d_info.  `bf16` (FP32 only): the SYRK
+// and GEMM updates on the BF16 tiles with split operands.
 template <class T>
-void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info);
+void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, bool bf16 = false);
 // WL / WU, when not null, are tri_inverse_build's inverses of L and L^T.
 template <class T>
 void potrs(hipStream_t s, int64_t n, int64_t nrhs, const T *L, int64_t lda, T *B, int64_t ldb,
@@ -107,7 +172,9 @@ struct RefineOut {
     double eta = 0.0;
 };
 struct LowFactor {
+    int precision = LMX_PREC_F32;  // LMX_PREC_F32 or LMX_PREC_BF16 (FP32 storage, BF16 updates)
     bool spd = false;
+    bool tournament = false;       // LU: tournament pivoting for this factor
     int64_t n = 0;
     const double *A = nullptr;  // the FP64 matrix (residuals); must outlive the factor's use
     int64_t lda = 0;

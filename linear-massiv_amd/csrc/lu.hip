@@ -6,8 +6,10 @@
 // consecutive rows held in registers.  Per column j:
 //   1. each workgroup finds its candidate max |a_ij| over rows i >= j (ties:
 //      lowest row, NaN never chosen -- the rule of rawPivotSearch,
-//      src/.../Internal/Kernel.hs:1541-1556): DPP wave reductions, then one
-//      LDS exchange between the waves;
+//      src/.../Internal/Kernel.hs:1541-1556; or, for the generic lu, the
+//      highest row among equal magnitudes -- maximumBy keeps the later of
+//      two equal elements, src/.../Solve/LU.hs:128-135): DPP wave
+//      reductions, then one LDS exchange between the waves;
 //   2. it publishes the candidate (value, row, full row data) in slot j%2,
 //      and workgroup 0 publishes row j, then raises its sequence number;
 //   3. every workgroup waits for all P sequence numbers, reduces the P
@@ -112,10 +114,21 @@ struct PanelLds {
     alignas(8) unsigned pay[2][PanelRec<T>::TOTAL];  // polled record payloads (several workgroups)
 };
 
-// The workgroup's best of the PW wave slots: largest value, lowest id on
-// ties.  All slots are loaded before any comparison: compared as they are
-// read, every step waits for its own LDS round trip.
-template <class T>
+// Tie rules of the pivot search: the lowest row among equal magnitudes
+// (TL = false, rawPivotSearch) or the highest (TL = true, lu's maximumBy).
+// A row id of "none" loses to every real row under either rule.
+template <bool TL>
+struct TieRule {
+    static constexpr int NONE = TL ? -1 : INT_MAX;
+    __device__ static int pick(int a, int b) { return TL ? max(a, b) : min(a, b); }
+    __device__ static bool better(int a, int b) { return TL ? a > b : a < b; }
+    __device__ static int wave_pick(int v) { return TL ? wave_max_int(v) : wave_min(v); }
+};
+
+// The workgroup's best of the PW wave slots: largest value, the tie rule's
+// id on ties.  All slots are loaded before any comparison: compared as they
+// are read, every step waits for its own LDS round trip.
+template <class T, bool TL = false>
 __device__ __forceinline__ void wg_select(const T *vals, const int *ids, T &best, int &bid, int &bw) {
     T v[PW];
     int id[PW];
@@ -129,7 +142,7 @@ __device__ __forceinline__ void wg_select(const T *vals, const int *ids, T &best
     bw = 0;
 #pragma unroll
     for (int x = 1; x < PW; ++x) {
-        const bool take = v[x] > best || (v[x] == best && id[x] < bid);
+        const bool take = v[x] > best || (v[x] == best && TieRule<TL>::better(id[x], bid));
         best = take ? v[x] : best;
         bid = take ? id[x] : bid;
         bw = take ? x : bw;
@@ -138,17 +151,18 @@ __device__ __forceinline__ void wg_select(const T *vals, const int *ids, T &best
 
 // One column (j = j0 + U) of the panel: pivot search, interchange, scaling
 // and rank-1 update of the rest of the panel.  Returns the pivot row.
-template <class T, int U>
+template <class T, int U, bool TL>
 __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, int j0, int kb, int m, int P,
                                              PanelSync<T> *__restrict__ sy, unsigned base, int k0,
                                              int32_t *__restrict__ ipiv, int &local_info, bool &failed) {
     typedef PanelRec<T> R;
+    typedef TieRule<TL> Tie;
     const int j = j0 + U;
     if (j >= kb) return;
     const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int par = j & 1;
     // 1. this wave's candidate: max |a_ij| over its rows i >= j, then the
-    //    lowest row attaining it; its owner writes the row to the wave slot
+    //    tie rule's row attaining it; its owner writes the row to the wave slot
     T a[RPT];
     T tmax = T(-1);
 #pragma unroll
@@ -157,11 +171,11 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
         tmax = fmax(tmax, a[q]);
     }
     const T wmax = wave_max(tmax);
-    int trow = INT_MAX;
+    int trow = Tie::NONE;
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
-        if (a[q] == wmax && a[q] >= T(0)) trow = min(trow, st.row[q]);
-    const int wrow = wave_min(trow);
+        if (a[q] == wmax && a[q] >= T(0)) trow = Tie::pick(trow, st.row[q]);
+    const int wrow = Tie::wave_pick(trow);
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         if (st.row[q] == wrow) {
@@ -183,10 +197,10 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     // 2. this workgroup's candidate
     T best;
     int bw, brow;
-    wg_select<T>(L.slot_val[par], L.slot_row[par], best, brow, bw);
+    wg_select<T, TL>(L.slot_val[par], L.slot_row[par], best, brow, bw);
     if (!(best >= T(0))) {
         bw = -1;
-        brow = INT_MAX;
+        brow = Tie::NONE;
     }
     int p, ptag, dtag;
     const T *prow;
@@ -264,17 +278,17 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
             }
         }
         __syncthreads();
-        // 5. the pivot: largest value over the workgroups, lowest row on
-        //    ties; every wave reduces the P records itself (lane x reads
-        //    record x), so no serial loop and no barrier
+        // 5. the pivot: largest value over the workgroups, the tie rule's
+        //    row on ties; every wave reduces the P records itself (lane x
+        //    reads record x), so no serial loop and no barrier
         const unsigned *pay = L.pay[par];
         const T xv = lane < P ? *reinterpret_cast<const T *>(pay + lane * R::W) : T(-1);
-        const int xr = lane < P ? (int)pay[lane * R::W + R::WPV] : INT_MAX;
+        const int xr = lane < P ? (int)pay[lane * R::W + R::WPV] : Tie::NONE;
         const T gb = wave_max(xv);
-        const int gr = wave_min((gb >= T(0) && xv == gb) ? xr : INT_MAX);
+        const int gr = Tie::wave_pick((gb >= T(0) && xv == gb) ? xr : Tie::NONE);
         // the workgroup that owns the pivot row (rows are split in
         // PANEL_ROWS-row blocks)
-        const int gw = gr == INT_MAX ? -1 : gr / PANEL_ROWS;
+        const int gw = gr == Tie::NONE ? -1 : gr / PANEL_ROWS;
         drow = reinterpret_cast<const T *>(pay + MAXP * R::W + R::DDATA);
         dtag = (int)pay[MAXP * R::W];
         if (gw < 0 || isnan(drow[U])) {
@@ -328,7 +342,7 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     }
 }
 
-template <class T>
+template <class T, bool TL>
 __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
                                                           int k0, int32_t *__restrict__ ipiv,
                                                           int32_t *__restrict__ srcmap,
@@ -352,10 +366,10 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
     bool failed = false;
     int groups = 0;
     for (int j0 = 0; j0 < kb; j0 += PG) {
-        panel_column<T, 0>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
-        panel_column<T, 1>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
-        panel_column<T, 2>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
-        panel_column<T, 3>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 0, TL>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 1, TL>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 2, TL>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
+        panel_column<T, 3, TL>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
         // rotate: the next group's first column moves to register 0
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
@@ -873,6 +887,8 @@ template <class T>
 constexpr size_t aux_gemm_lds_footprint() {
     return sizeof(T) == 4 ? 64000 : 0;
 }
+// The BF16 lookahead GEMM's workgroup (80 KB of tiles) held to one per CU.
+constexpr size_t BF16_SHARING_LDS = 96 * 1024;
 template <class T>
 __global__ __launch_bounds__(256) void row_gather_multi_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
                                                                const int32_t *__restrict__ pairs, int npanels) {
@@ -924,20 +940,32 @@ __global__ void finish_kernel(const unsigned *err, int32_t *info) {
 }
 
 // Swap list -> permutation vector: perm[i] = original row at position i.
-// One thread, the vector in LDS (n <= 16384).
-__global__ void ipiv_to_perm_kernel(const int32_t *__restrict__ ipiv, int32_t *__restrict__ perm, int n) {
-    extern __shared__ int32_t sp[];
+// The swaps are sequential (step k reads what step k' < k left at ipiv[k]),
+// so one lane applies them to the vector held in LDS (n <= PERM_MAX_N);
+// wave 0 loads the swap list 64 entries at a time and hands each entry to
+// that lane by v_readlane, so no step waits for a global load.
+constexpr int PERM_MAX_N = 32768;
+__global__ __launch_bounds__(256) void ipiv_to_perm_kernel(const int32_t *__restrict__ ipiv, int32_t *__restrict__ perm,
+                                                           int n) {
+    __shared__ int32_t sp[PERM_MAX_N];
     for (int i = threadIdx.x; i < n; i += blockDim.x) sp[i] = i;
     __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < n; ++k) {
-            int p = ipiv[k];
-            if (p != k && p >= 0 && p < n) {
-                int a = sp[k];
-                sp[k] = sp[p];
-                sp[p] = a;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int k0 = 0; k0 < n; k0 += 64) {
+            const int chunk = ipiv[k0 + lane < n ? k0 + lane : n - 1];
+            const int cnt = min(64, n - k0);
+            for (int kk = 0; kk < cnt; ++kk) {
+                const int k = k0 + kk;
+                const int p = __builtin_amdgcn_readlane(chunk, kk);
+                if (lane == 0 && p != k && p >= 0 && p < n) {
+                    const int a = sp[k], b = sp[p];
+                    sp[k] = b;
+                    sp[p] = a;
+                }
             }
         }
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
 }
@@ -967,7 +995,7 @@ __global__ void set_i32_kernel(int32_t *p, int32_t v) {
 
 template <class T>
 void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32_t *ipiv, int32_t *srcmap,
-                  int32_t *moves, int32_t *info, PanelSync<T> *sy, unsigned &base) {
+                  int32_t *moves, int32_t *info, PanelSync<T> *sy, unsigned &base, bool tie_last) {
     int P = (int)cdiv(m, PANEL_ROWS);
     if (P > MAXP) fail_arg("getrf: matrix too tall for the panel kernel");
     TimedRegion tr("getrf_panel", s);
@@ -980,8 +1008,12 @@ void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32
     // hardware queue of its own: each hand-over between it and the stream's
     // other kernels cost about 11 us (about 10 ms per FP32 factorisation at
     // n = 16384).
-    hipLaunchKernelGGL(getf2_panel_kernel<T>, dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap, moves, info,
-                       sy, P, base);
+    if (tie_last)
+        hipLaunchKernelGGL((getf2_panel_kernel<T, true>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
+                           moves, info, sy, P, base);
+    else
+        hipLaunchKernelGGL((getf2_panel_kernel<T, false>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
+                           moves, info, sy, P, base);
     LMX_LAUNCH_CHECK();
     base += (unsigned)kb;  // sequence numbers stay monotonic across launches
 }
@@ -1037,12 +1069,34 @@ void clear_info(hipStream_t s, int32_t *d_info) {
 }
 
 // getrf with optional permutation-vector output (d_perm may be null).
+// A22 -= L21 U12 for the trailing updates: FP32 or FP64 MFMA tiles, or, for
+// the BF16 factor (bf16 = true, FP32 storage), the BF16 tiles with each
+// operand split in two BF16 parts.  A nonzero lds_footprint holds the
+// workgroups per CU down so that panel workgroups find room beside them.
+template <class T>
+void trailing_update(hipStream_t s, bool bf16, int64_t m, int64_t n, int64_t k, const T *L21, int64_t lda,
+                     const T *U12, T *A22, size_t lds_footprint) {
+    if constexpr (sizeof(T) == 4) {
+        if (bf16) {
+            gemm_bf16x3(s, false, false, m, n, k, -1.0f, L21, lda, U12, lda, 1.0f, A22, lda, false,
+                        lds_footprint ? BF16_SHARING_LDS : 0);
+            return;
+        }
+    }
+    if (lds_footprint)
+        gemm_sharing<T>(s, m, n, k, T(-1), L21, lda, U12, lda, T(1), A22, lda, lds_footprint);
+    else
+        gemm(s, false, false, m, n, k, T(-1), L21, lda, U12, lda, T(1), A22, lda);
+}
+
 template <class T>
 void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, int32_t *d_perm,
-                int32_t *d_info, bool tournament) {
+                int32_t *d_info, bool tournament, bool tie_last, bool bf16) {
     if (n < 0 || lda < std::max<int64_t>(1, n)) fail_arg("getrf: bad dimension or stride");
     if (n > INT32_MAX / 2) fail_arg("getrf: n too large");
     if (tournament && sizeof(T) != 4) fail_arg("getrf: tournament pivoting is for the FP32 factor");
+    if (tournament && tie_last) fail_arg("getrf: the tie rule of lu needs column-by-column pivoting");
+    if (bf16 && sizeof(T) != 4) fail_arg("getrf: BF16 updates are for the FP32-stored factor");
     clear_info(s, d_info);
     if (n == 0) return;
     TimedRegion trall("getrf", s, 2.0 / 3.0 * (double)n * (double)n * (double)n);
@@ -1097,9 +1151,9 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
                 if (tournament)
                     launch_tslu<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, ts);
                 else
-                    launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
+                    launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base, tie_last);
             } else {
-                launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base);
+                launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base, tie_last);
             }
             // this outer panel's columns now, the rest later (move list kept)
             gather_rows<T>(s, A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, moves, srcmap,
@@ -1134,8 +1188,8 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
         if (trail > 0) {
             gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
             tri_inverse_solve<T>(s, true, false, nbw, narrow, A11, lda, W, A12, lda);
-            gemm(s, false, false, trail, narrow, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12, lda, T(1),
-                 A12 + (int64_t)nbw * lda, lda);
+            trailing_update<T>(s, bf16, trail, narrow, nbw, A11 + (int64_t)nbw * lda, lda, A12,
+                               A12 + (int64_t)nbw * lda, 0);
         }
         LMX_HIP(hipEventRecord(evE, s));
         LMX_HIP(hipStreamWaitEvent(aux, evE, 0));
@@ -1144,8 +1198,8 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             tri_inverse_solve<T>(aux, true, false, nbw, trail - narrow, A11, lda, W, A12 + narrow, lda);
             // at most two GEMM workgroups per CU, so every CU keeps room for
             // a panel workgroup of the caller's stream
-            gemm_sharing<T>(aux, trail, trail - narrow, nbw, T(-1), A11 + (int64_t)nbw * lda, lda, A12 + narrow, lda,
-                            T(1), A12 + (int64_t)nbw * lda + narrow, lda, aux_gemm_lds_footprint<T>());
+            trailing_update<T>(aux, bf16, trail, trail - narrow, nbw, A11 + (int64_t)nbw * lda, lda, A12 + narrow,
+                               A12 + (int64_t)nbw * lda + narrow, aux_gemm_lds_footprint<T>());
         }
         LMX_HIP(hipEventRecord(evF, aux));
         pending = true;
@@ -1187,18 +1241,19 @@ template <class T>
 void getrs(hipStream_t s, int64_t n, int64_t nrhs, const T *LU, int64_t lda, const int32_t *d_ipiv, T *B,
            int64_t ldb) {
     if (n == 0 || nrhs == 0) return;
-    if (n > 16384) fail_arg("getrs_dev: swap-list conversion supports n <= 16384 (use gesv_dev)");
+    if (n > PERM_MAX_N) fail_arg("getrs_dev: n above 32768 (the largest order getrf factors)");
     Scratch perm((size_t)n * sizeof(int32_t), s);
-    hipLaunchKernelGGL(ipiv_to_perm_kernel, dim3(1), dim3(256), (size_t)n * sizeof(int32_t), s, d_ipiv,
-                       perm.as<int32_t>(), (int)n);
+    hipLaunchKernelGGL(ipiv_to_perm_kernel, dim3(1), dim3(256), 0, s, d_ipiv, perm.as<int32_t>(), (int)n);
     LMX_LAUNCH_CHECK();
     getrs_perm<T>(s, n, nrhs, LU, lda, perm.as<int32_t>(), B, ldb);
 }
 
 template void getrf<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *);
 template void getrf<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *);
-template void getrf_impl<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *, int32_t *, bool);
-template void getrf_impl<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *, int32_t *, bool);
+template void getrf_impl<double>(hipStream_t, int64_t, double *, int64_t, int32_t *, int32_t *, int32_t *, bool,
+                                 bool, bool);
+template void getrf_impl<float>(hipStream_t, int64_t, float *, int64_t, int32_t *, int32_t *, int32_t *, bool,
+                                bool, bool);
 template void getrs<double>(hipStream_t, int64_t, int64_t, const double *, int64_t, const int32_t *, double *,
                             int64_t);
 template void getrs<float>(hipStream_t, int64_t, int64_t, const float *, int64_t, const int32_t *, float *,
@@ -1255,9 +1310,56 @@ int getrf_host(int64_t n, T *A, int64_t lda, int32_t *ipiv) {
     });
 }
 
+// The generic lu: factors with the tie rule of maximumBy and returns the
+// permutation vector instead of the swap list.
+template <class T>
+void lu_dev(hipStream_t s, int64_t n, T *dA, int64_t lda, int32_t *d_perm, int32_t *d_info) {
+    Scratch ipiv((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t), s);
+    getrf_impl<T>(s, n, dA, lda, ipiv.as<int32_t>(), d_perm, d_info, false, true);
+}
+
+template <class T>
+int lu_host(int64_t n, T *A, int64_t lda, int32_t *perm) {
+    return guarded([&] {
+        check_square(n, lda, "lu");
+        if (n == 0) return LMX_OK;
+        Context &c = ctx();
+        std::lock_guard<std::mutex> g(c.host_mu);
+        size_t ab = extent(n, n, lda) / sizeof(double) * sizeof(T);
+        T *dA = (T *)staging(c, 0, ab);
+        int32_t *dp = (int32_t *)staging(c, 1, (size_t)n * 4 + 64);
+        int32_t *di = dp + n;
+        LMX_HIP(hipMemcpyAsync(dA, A, ab, hipMemcpyHostToDevice, c.stream));
+        lu_dev<T>(c.stream, n, dA, lda, dp, di);
+        LMX_HIP(hipMemcpyAsync(A, dA, ab, hipMemcpyDeviceToHost, c.stream));
+        LMX_HIP(hipMemcpyAsync(perm, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+        return read_info(c.stream, di);
+    });
+}
+
 }  // namespace
 
 extern "C" {
+
+int lmx_dlu(int64_t n, double *A, int64_t lda, int32_t *perm) { return lu_host<double>(n, A, lda, perm); }
+
+int lmx_slu(int64_t n, float *A, int64_t lda, int32_t *perm) { return lu_host<float>(n, A, lda, perm); }
+
+int lmx_dlu_dev(void *stream, int64_t n, double *dA, int64_t lda, int32_t *d_perm, int32_t *d_info) {
+    return guarded([&] {
+        check_square(n, lda, "lu");
+        lu_dev<double>(resolve_stream(stream), n, dA, lda, d_perm, d_info);
+        return LMX_OK;
+    });
+}
+
+int lmx_slu_dev(void *stream, int64_t n, float *dA, int64_t lda, int32_t *d_perm, int32_t *d_info) {
+    return guarded([&] {
+        check_square(n, lda, "lu");
+        lu_dev<float>(resolve_stream(stream), n, dA, lda, d_perm, d_info);
+        return LMX_OK;
+    });
+}
 
 int lmx_dgetrf_dev(void *stream, int64_t n, double *dA, int64_t lda, int32_t *d_ipiv, int32_t *d_info) {
     return guarded([&] {

@@ -11,9 +11,11 @@
 // or eta stops shrinking (less than 10% per step), or 30 steps.  A solve that does not converge is
 // redone with the FP64 factorisation (fell_back = 1).  Only eta (one double)
 // crosses to the host per step.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "common.hpp"
 #include "device.hpp"
@@ -25,257 +27,6 @@ namespace {
 
 constexpr int ITMAX = 30;
 constexpr double EPS64 = 2.220446049250313e-16;
-
-__global__ void d2f_kernel(const double *__restrict__ in, int64_t ldi, float *__restrict__ out, int64_t ldo,
-                           int64_t rows, int64_t cols) {
-    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= rows * cols) return;
-    int64_t i = e / cols, c = e % cols;
-    out[i * ldo + c] = (float)in[i * ldi + c];
-}
-
-// X += (double) D ; optionally X = D when `assign`.
-__global__ void f2d_acc_kernel(const float *__restrict__ D, int64_t ldd, double *__restrict__ X, int64_t ldx,
-                               int64_t rows, int64_t cols, int assign) {
-    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= rows * cols) return;
-    int64_t i = e / cols, c = e % cols;
-    double d = (double)D[i * ldd + c];
-    X[i * ldx + c] = assign ? d : X[i * ldx + c] + d;
-}
-
-__global__ void copy_d_kernel(const double *__restrict__ in, int64_t ldi, double *__restrict__ out, int64_t ldo,
-                              int64_t rows, int64_t cols) {
-    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= rows * cols) return;
-    int64_t i = e / cols, c = e % cols;
-    out[i * ldo + c] = in[i * ldi + c];
-}
-
-__global__ void zero_d_kernel(double *p, int n) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0.0;
-}
-
-// Non-negative doubles order like their bit patterns: max via integer atomics.
-__device__ inline void atomic_max_nonneg(double *addr, double v) {
-    if (!(v >= 0.0)) v = __builtin_huge_val();  // NaN poisons the norm
-    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
-}
-
-// ||A||_inf = max_i sum_j |a_ij|: one wave per row at a time (coalesced
-// row reads, four partial sums per lane), rows strided over the grid, one
-// atomic per workgroup.
-__global__ __launch_bounds__(256) void norm_inf_rows_kernel(const double *__restrict__ A, int64_t lda,
-                                                            int64_t n, double *__restrict__ out) {
-    __shared__ double part[4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t nw = (int64_t)gridDim.x * 4;
-    double best = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < n; i += nw) {
-        const double *row = A + i * lda;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int64_t j = lane;
-        for (; j + 192 < n; j += 256) {
-            s0 += fabs(row[j]);
-            s1 += fabs(row[j + 64]);
-            s2 += fabs(row[j + 128]);
-            s3 += fabs(row[j + 192]);
-        }
-        for (; j < n; j += 64) s0 += fabs(row[j]);
-        const double sum = wave_sum((s0 + s1) + (s2 + s3));
-        best = (sum > best || isnan(sum)) ? sum : best;
-    }
-    if (lane == 0) part[wave] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = part[0];
-        for (int x = 1; x < 4; ++x) m = (part[x] > m || isnan(part[x])) ? part[x] : m;
-        atomic_max_nonneg(out, m);
-    }
-}
-
-// out[c] = max_i |X[i][c]| for an n x ncols block: a workgroup covers
-// COLMAX_ROWS rows and 64 columns (blockIdx.y), reduces in LDS, and issues
-// one atomic per column.
-constexpr int COLMAX_ROWS = 1024;
-__global__ __launch_bounds__(256) void colmax_kernel(const double *__restrict__ X, int64_t ldx, int64_t n,
-                                                     int ncols, double *__restrict__ out) {
-    __shared__ double part[4][64];
-    const int t = threadIdx.x, cl = t & 63, q = t >> 6;
-    const int c = blockIdx.y * 64 + cl;
-    const int64_t r0 = (int64_t)blockIdx.x * COLMAX_ROWS;
-    double m = 0.0;
-    if (c < ncols)
-        for (int64_t i = r0 + q; i < n && i < r0 + COLMAX_ROWS; i += 4) {
-            double v = fabs(X[i * ldx + c]);
-            if (!(v <= m)) m = v;  // keeps NaN
-        }
-    part[q][cl] = m;
-    __syncthreads();
-    if (q == 0 && c < ncols) {
-        double r = part[0][cl];
-        for (int x = 1; x < 4; ++x)
-            if (!(part[x][cl] <= r)) r = part[x][cl];
-        atomic_max_nonneg(&out[c], r);
-    }
-}
-
-// The same for at most 8 columns: one thread per row (16 rows per thread,
-// coalesced), wave and workgroup reductions, one atomic per column and
-// workgroup.  NaN becomes +inf first (as the atomic would make it).
-constexpr int COLMAX_FEW = 8;
-__global__ __launch_bounds__(256) void colmax_few_kernel(const double *__restrict__ X, int64_t ldx, int64_t n,
-                                                         int ncols, double *__restrict__ out) {
-    __shared__ double part[4][COLMAX_FEW];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    double m[COLMAX_FEW];
-#pragma unroll
-    for (int c = 0; c < COLMAX_FEW; ++c) m[c] = 0.0;
-    const int64_t base = (int64_t)blockIdx.x * 4096;
-#pragma unroll 4
-    for (int k = 0; k < 16; ++k) {
-        const int64_t i = base + k * 256 + t;
-        if (i < n) {
-#pragma unroll
-            for (int c = 0; c < COLMAX_FEW; ++c)
-                if (c < ncols) {
-                    double v = fabs(X[i * ldx + c]);
-                    if (isnan(v)) v = __builtin_huge_val();
-                    m[c] = fmax(m[c], v);
-                }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < COLMAX_FEW; ++c) {
-        const double w = wave_max(m[c]);
-        if (lane == 0) part[wave][c] = w;
-    }
-    __syncthreads();
-    if (t < ncols) atomic_max_nonneg(&out[t], fmax(fmax(part[0][t], part[1][t]), fmax(part[2][t], part[3][t])));
-}
-
-// R = B - A X in FP64 for nrhs <= NR: each wave takes RESID_RPW rows of A
-// and walks them together, so one load of x[k] serves all of them (coalesced
-// row reads).  The columns go in pairs of chunks of RESID_CH * 64, pipelined
-// one chunk ahead in two register buffers used in turn: a chunk's products
-// wait only for that chunk while the next one loads.  The loop has no exit
-// between the two halves of a pair (an exit there lets the compiler move the
-// prefetch down to its use); columns past the last full pair go through a
-// plain tail loop.  The prefetch chunk is clamped to the last full one, so
-// every load is in bounds.  A wave's rows past n repeat
-// row n - 1 (in bounds) and are not stored.  Wave-shuffle reduction.
-constexpr int RESID_RPW = 4;
-constexpr int RESID_CH = 2;
-
-template <int NR>
-struct ResidChunk {
-    double a[RESID_CH][RESID_RPW], x[RESID_CH][NR];
-};
-
-template <int NR>
-__device__ __forceinline__ void resid_load(ResidChunk<NR> &b, const double *const (&a)[RESID_RPW],
-                                           const double *__restrict__ X, int64_t ldx, int nrhs, int64_t k0,
-                                           int lane) {
-#pragma unroll
-    for (int u = 0; u < RESID_CH; ++u) {
-        const int64_t k = k0 + 64 * u + lane;
-#pragma unroll
-        for (int r = 0; r < RESID_RPW; ++r) b.a[u][r] = a[r][k];
-#pragma unroll
-        for (int c = 0; c < NR; ++c) b.x[u][c] = X[k * ldx + (c < nrhs ? c : 0)];
-    }
-}
-
-template <int NR>
-__device__ __forceinline__ void resid_fma(const ResidChunk<NR> &b, double (&acc)[RESID_RPW][NR]) {
-#pragma unroll
-    for (int u = 0; u < RESID_CH; ++u)
-#pragma unroll
-        for (int r = 0; r < RESID_RPW; ++r)
-#pragma unroll
-            for (int c = 0; c < NR; ++c) acc[r][c] = fma(b.a[u][r], b.x[u][c], acc[r][c]);
-}
-
-template <int NR>
-__global__ __launch_bounds__(256) void resid_kernel(int64_t n, int nrhs, const double *__restrict__ A, int64_t lda,
-                                                    const double *__restrict__ X, int64_t ldx,
-                                                    const double *__restrict__ B, int64_t ldb,
-                                                    double *__restrict__ R, int64_t ldr) {
-    constexpr int RW = RESID_RPW, CW = RESID_CH * 64;
-    const int lane = threadIdx.x & 63;
-    const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
-    if (i0 >= n) return;  // wave-uniform
-    const double *a[RW];
-#pragma unroll
-    for (int r = 0; r < RW; ++r) a[r] = A + (i0 + r < n ? i0 + r : n - 1) * lda;
-    double acc[RW][NR];
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int c = 0; c < NR; ++c) acc[r][c] = 0.0;
-    const int64_t nfull = n / (2 * CW) * (2 * CW), klast = nfull - CW;
-    if (nfull > 0) {
-        ResidChunk<NR> b0, b1;
-        resid_load<NR>(b0, a, X, ldx, nrhs, 0, lane);
-        for (int64_t k0 = 0; k0 < nfull; k0 += 2 * CW) {
-            resid_load<NR>(b1, a, X, ldx, nrhs, k0 + CW, lane);
-            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
-            resid_fma<NR>(b0, acc);
-            resid_load<NR>(b0, a, X, ldx, nrhs, k0 + 2 * CW < klast ? k0 + 2 * CW : klast, lane);
-            __builtin_amdgcn_sched_barrier(0);
-            resid_fma<NR>(b1, acc);
-        }
-    }
-    for (int64_t k = nfull + lane; k < n; k += 64) {
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const double av = a[r][k];
-#pragma unroll
-            for (int c = 0; c < NR; ++c) acc[r][c] = fma(av, X[k * ldx + (c < nrhs ? c : 0)], acc[r][c]);
-        }
-    }
-    double out = 0.0;
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int c = 0; c < NR; ++c) {
-            double v = acc[r][c];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-            out = lane == r * NR + c ? v : out;
-        }
-    const int r = lane / NR, c = lane % NR;
-    if (r < RW && i0 + r < n && c < nrhs) R[(i0 + r) * ldr + c] = B[(i0 + r) * ldb + c] - out;
-}
-
-void colmax(hipStream_t s, const double *X, int64_t ldx, int64_t n, int ncols, double *out) {
-    if (ncols <= COLMAX_FEW) {
-        hipLaunchKernelGGL(colmax_few_kernel, dim3((unsigned)cdiv(n, 4096)), dim3(256), 0, s, X, ldx, n, ncols, out);
-        LMX_LAUNCH_CHECK();
-        return;
-    }
-    dim3 g((unsigned)cdiv(n, COLMAX_ROWS), (unsigned)cdiv(ncols, 64));
-    hipLaunchKernelGGL(colmax_kernel, g, dim3(256), 0, s, X, ldx, n, ncols, out);
-    LMX_LAUNCH_CHECK();
-}
-
-// R = B - A X (FP64).
-void residual(hipStream_t s, int64_t n, int64_t k, const double *A, int64_t lda, const double *X, int64_t ldx,
-              const double *B, int64_t ldb, double *R, int64_t ldr) {
-    if (k <= 4) {
-        const dim3 g((unsigned)cdiv(n, 4 * RESID_RPW));
-        if (k == 1)
-            hipLaunchKernelGGL(resid_kernel<1>, g, dim3(256), 0, s, n, 1, A, lda, X, ldx, B, ldb, R, ldr);
-        else
-            hipLaunchKernelGGL(resid_kernel<4>, g, dim3(256), 0, s, n, (int)k, A, lda, X, ldx, B, ldb, R, ldr);
-        LMX_LAUNCH_CHECK();
-        return;
-    }
-    hipLaunchKernelGGL(copy_d_kernel, dim3((unsigned)cdiv(n * k, 256)), dim3(256), 0, s, B, ldb, R, ldr, n, k);
-    LMX_LAUNCH_CHECK();
-    gemm(s, false, false, n, k, n, -1.0, A, lda, X, ldx, 1.0, R, ldr);
-}
 
 // norms layout: [0] ||A||, [1..1+k) ||b_c||, [1+k..1+2k) ||x_c||, [1+2k..1+3k) ||r_c||,
 // [1+3k] eta, [2+3k] all-columns-pass flag (1.0 / 0.0)
@@ -293,11 +44,265 @@ __global__ void eta_kernel(double *norms, int k, double crit) {
     norms[2 + 3 * k] = pass;
 }
 
-template <class F>
-void launch1d(hipStream_t s, int64_t elems, F launch) {
-    if (elems > 0) launch(dim3((unsigned)cdiv(elems, 256)));
+
+
+// ---------------------------------------------------------------- GMRES-IR
+//
+// Refinement with GMRES on the left-preconditioned system M^{-1} A d =
+// M^{-1} r (M = the low-precision factor), for factors too coarse for plain
+// refinement to contract (the BF16 factor at large kappa): Carson & Higham,
+// SIAM J. Sci. Comput. 40(2), 2018.  Every right-hand-side column has its
+// own Krylov space; the columns advance together, so each step is one FP64
+// product A V_j (n x k), one factor solve of the n x k block and classical
+// Gram-Schmidt with reorthogonalisation, all on the device.  The Hessenberg
+// columns (2 (j + 1) k + k doubles per step) come back to the host for the
+// Givens rotations.  Outer steps recompute r = b - A x in FP64 and test
+// convergence with the same rule as plain refinement.
+constexpr int GM_COLS = 64;      // right-hand sides per Krylov group
+constexpr int GM_RESTART = 40;   // basis vectors per outer step
+constexpr int GM_OUTER = 12;     // outer steps
+constexpr double GM_INNER_TOL = 1e-10;
+constexpr int GM_CHUNK = 1024;   // rows per partial dot product
+
+// part[(i * nchunk + chunk) * k + c] = sum over the chunk's rows of
+// V_i[r][c] W[r][c] (V_i = V + i vstride, row stride k); rows in a fixed
+// order, then a fixed tree: the sums do not vary by run.
+__global__ __launch_bounds__(256) void coldots_partial_kernel(const double *__restrict__ V, int64_t vstride,
+                                                              const double *__restrict__ W, int64_t n, int k,
+                                                              double *__restrict__ part) {
+    __shared__ double red[4][64];
+    const int c = threadIdx.x & 63, rsub = threadIdx.x >> 6;
+    const int i = blockIdx.x, chunk = blockIdx.y;
+    const double *v = V + (int64_t)i * vstride;
+    const int64_t r0 = (int64_t)chunk * GM_CHUNK;
+    double s = 0.0;
+    if (c < k)
+        for (int64_t r = r0 + rsub; r < n && r < r0 + GM_CHUNK; r += 4) s = fma(v[r * k + c], W[r * k + c], s);
+    red[rsub][c] = s;
+    __syncthreads();
+    if (rsub == 0 && c < k)
+        part[((int64_t)i * gridDim.y + chunk) * k + c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
 
+// out[i * k + c] = sum over chunks (in order) of the partials.
+__global__ void coldots_finish_kernel(const double *__restrict__ part, int nvec, int nchunk, int k,
+                                      double *__restrict__ out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nvec * k) return;
+    const int i = e / k, c = e % k;
+    double s = 0.0;
+    for (int q = 0; q < nchunk; ++q) s += part[((int64_t)i * nchunk + q) * k + c];
+    out[e] = s;
+}
+
+// W[r][c] += sign * sum_i h[i][c] V_i[r][c] (W row stride ldw).
+__global__ void block_axpy_kernel(double *__restrict__ W, int64_t ldw, const double *__restrict__ V,
+                                  int64_t vstride, const double *__restrict__ h, int nvec, int64_t n, int k,
+                                  double sign) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * k) return;
+    const int64_t r = e / k;
+    const int c = (int)(e % k);
+    double acc = 0.0;
+    for (int i = 0; i < nvec; ++i) acc = fma(h[i * k + c], V[i * vstride + e], acc);
+    W[r * ldw + c] += sign * acc;
+}
+
+// dst[r][c] = src[r][c] / sqrt(nrm2sq[c]) (0 when the norm is 0).
+__global__ void col_normalise_kernel(double *__restrict__ dst, const double *__restrict__ src,
+                                     const double *__restrict__ nrm2sq, int64_t n, int k) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * k) return;
+    const double d = sqrt(nrm2sq[e % k]);
+    dst[e] = d > 0.0 ? src[e] / d : 0.0;
+}
+
+struct GmresBuffers {
+    double *V, *W, *R, *part, *dots;
+    float *R32;
+};
+
+void coldots(hipStream_t s, const double *V, int64_t vstride, int nvec, const double *W, int64_t n, int k,
+             double *part, double *out) {
+    const int nchunk = (int)cdiv(n, GM_CHUNK);
+    hipLaunchKernelGGL(coldots_partial_kernel, dim3(nvec, nchunk), dim3(256), 0, s, V, vstride, W, n, k, part);
+    hipLaunchKernelGGL(coldots_finish_kernel, dim3((unsigned)cdiv((int64_t)nvec * k, 256)), dim3(256), 0, s, part,
+                       nvec, nchunk, k, out);
+    LMX_LAUNCH_CHECK();
+}
+
+// Y (n x k, stride k) = M^{-1} Y with the low-precision factor, through the
+// FP32 staging block.
+void precondition(hipStream_t s, const LowFactor &f, int64_t k, double *Y, float *Y32) {
+    d2f(s, Y, k, Y32, k, f.n, k);
+    if (f.spd)
+        potrs<float>(s, f.n, k, f.A32, f.n, Y32, k, f.WL, f.WU);
+    else
+        getrs_perm<float>(s, f.n, k, f.A32, f.n, f.perm, Y32, k, f.WL, f.WU);
+    f2d_acc(s, Y32, k, Y, k, f.n, k, true);
+}
+
+// One group of k <= GM_COLS columns.  Returns the outer steps taken and sets
+// *conv when every column met the criterion.
+int gmres_group(hipStream_t s, const LowFactor &f, int k, const double *B, int64_t ldb, double *X, int64_t ldx,
+                const GmresBuffers &g, double *norms, double crit, double *eta_out, bool *conv, int *inner_total) {
+    const int64_t n = f.n, nk = n * k;
+    const int K = k, M = GM_RESTART;
+    std::vector<double> H((size_t)K * (M + 1) * M), cs((size_t)K * M), sn((size_t)K * M), gv((size_t)K * (M + 1)),
+        beta(K), y((size_t)M), hbuf((size_t)(2 * (M + 1) + 1) * K);
+    auto Hc = [&](int c, int i, int j) -> double & { return H[((size_t)c * (M + 1) + i) * M + j]; };
+    zero_d(s, norms, 3 * K + 8);
+    copy_d(s, f.anorm, 1, norms, 1, 1, 1);
+    colmax(s, B, ldb, n, K, norms + 1);
+    // X = 0
+    zero_d(s, g.R, nk);
+    copy_d(s, g.R, k, X, ldx, n, k);
+    *conv = false;
+    double prev = INFINITY;
+    int outer = 0;
+    for (; outer < GM_OUTER; ++outer) {
+        residual(s, n, k, f.A, f.lda, X, ldx, B, ldb, g.R, k);
+        if (outer > 0) {
+            zero_d(s, norms + 1 + K, 2 * K + 2);
+            colmax(s, X, ldx, n, K, norms + 1 + K);
+            colmax(s, g.R, k, n, K, norms + 1 + 2 * K);
+            hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
+            LMX_LAUNCH_CHECK();
+            double h[2];
+            LMX_HIP(hipMemcpyAsync(h, norms + 1 + 3 * K, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            LMX_HIP(hipStreamSynchronize(s));
+            *eta_out = h[0];
+            if (h[1] == 1.0) {
+                *conv = true;
+                return outer;
+            }
+            if (!(h[0] < 0.9 * prev)) return outer;  // stagnation (or NaN)
+            prev = h[0];
+        }
+        // v_0 = M^{-1} r / ||M^{-1} r||
+        copy_d(s, g.R, k, g.V, k, n, k);
+        precondition(s, f, k, g.V, g.R32);
+        coldots(s, g.V, nk, 1, g.V, n, k, g.part, g.dots);
+        LMX_HIP(hipMemcpyAsync(beta.data(), g.dots, K * sizeof(double), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(col_normalise_kernel, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, s, g.V, g.V, g.dots, n,
+                           k);
+        LMX_LAUNCH_CHECK();
+        LMX_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < K; ++c) {
+            beta[c] = std::sqrt(beta[c]);
+            std::fill(&gv[(size_t)c * (M + 1)], &gv[(size_t)c * (M + 1)] + M + 1, 0.0);
+            gv[(size_t)c * (M + 1)] = beta[c];
+        }
+        int jdone = 0;
+        for (int j = 0; j < M; ++j) {
+            double *Vj = g.V + (int64_t)j * nk, *Vn = g.V + (int64_t)(j + 1) * nk;
+            // W = M^{-1} A v_j
+            matmul_block(s, n, n, k, f.A, f.lda, Vj, k, g.W, k);
+            precondition(s, f, k, g.W, g.R32);
+            // classical Gram-Schmidt, twice
+            double *h1 = g.dots, *h2 = g.dots + (int64_t)(j + 1) * K, *hn = h2 + (int64_t)(j + 1) * K;
+            coldots(s, g.V, nk, j + 1, g.W, n, k, g.part, h1);
+            hipLaunchKernelGGL(block_axpy_kernel, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, s, g.W, k, g.V, nk, h1,
+                               j + 1, n, k, -1.0);
+            coldots(s, g.V, nk, j + 1, g.W, n, k, g.part, h2);
+            hipLaunchKernelGGL(block_axpy_kernel, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, s, g.W, k, g.V, nk, h2,
+                               j + 1, n, k, -1.0);
+            coldots(s, g.W, nk, 1, g.W, n, k, g.part, hn);
+            hipLaunchKernelGGL(col_normalise_kernel, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, s, Vn, g.W, hn, n,
+                               k);
+            LMX_LAUNCH_CHECK();
+            const size_t cnt = (size_t)(2 * (j + 1) + 1) * K;
+            LMX_HIP(hipMemcpyAsync(hbuf.data(), g.dots, cnt * sizeof(double), hipMemcpyDeviceToHost, s));
+            LMX_HIP(hipStreamSynchronize(s));
+            ++*inner_total;
+            bool all = true;
+            for (int c = 0; c < K; ++c) {
+                for (int i = 0; i <= j; ++i)
+                    Hc(c, i, j) = hbuf[(size_t)i * K + c] + hbuf[(size_t)(j + 1 + i) * K + c];
+                Hc(c, j + 1, j) = std::sqrt(hbuf[(size_t)(2 * (j + 1)) * K + c]);
+                double *csc = &cs[(size_t)c * M], *snc = &sn[(size_t)c * M], *gc = &gv[(size_t)c * (M + 1)];
+                for (int i = 0; i < j; ++i) {
+                    const double t = csc[i] * Hc(c, i, j) + snc[i] * Hc(c, i + 1, j);
+                    Hc(c, i + 1, j) = -snc[i] * Hc(c, i, j) + csc[i] * Hc(c, i + 1, j);
+                    Hc(c, i, j) = t;
+                }
+                const double d = std::hypot(Hc(c, j, j), Hc(c, j + 1, j));
+                csc[j] = d > 0.0 ? Hc(c, j, j) / d : 1.0;
+                snc[j] = d > 0.0 ? Hc(c, j + 1, j) / d : 0.0;
+                Hc(c, j, j) = d;
+                Hc(c, j + 1, j) = 0.0;
+                gc[j + 1] = -snc[j] * gc[j];
+                gc[j] = csc[j] * gc[j];
+                if (!(std::fabs(gc[j + 1]) <= GM_INNER_TOL * beta[c])) all = false;
+            }
+            jdone = j + 1;
+            if (all) break;
+        }
+        // x += V y, y = H^{-1} g per column (upper triangular, jdone x jdone)
+        std::vector<double> ycol((size_t)jdone * K);
+        for (int c = 0; c < K; ++c) {
+            const double *gc = &gv[(size_t)c * (M + 1)];
+            for (int i = jdone - 1; i >= 0; --i) {
+                double t = gc[i];
+                for (int l = i + 1; l < jdone; ++l) t -= Hc(c, i, l) * y[l];
+                y[i] = Hc(c, i, i) != 0.0 ? t / Hc(c, i, i) : 0.0;
+            }
+            for (int i = 0; i < jdone; ++i) ycol[(size_t)i * K + c] = y[i];
+        }
+        LMX_HIP(hipMemcpyAsync(g.dots, ycol.data(), ycol.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(block_axpy_kernel, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, s, X, ldx, g.V, nk, g.dots,
+                           jdone, n, k, 1.0);
+        LMX_LAUNCH_CHECK();
+        LMX_HIP(hipStreamSynchronize(s));  // ycol is a host buffer
+    }
+    // the last update's residual
+    residual(s, n, k, f.A, f.lda, X, ldx, B, ldb, g.R, k);
+    zero_d(s, norms + 1 + K, 2 * K + 2);
+    colmax(s, X, ldx, n, K, norms + 1 + K);
+    colmax(s, g.R, k, n, K, norms + 1 + 2 * K);
+    hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
+    LMX_LAUNCH_CHECK();
+    double h[2];
+    LMX_HIP(hipMemcpyAsync(h, norms + 1 + 3 * K, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    LMX_HIP(hipStreamSynchronize(s));
+    *eta_out = h[0];
+    *conv = h[1] == 1.0;
+    return outer;
+}
+
+// GMRES-IR over the right-hand sides in groups of GM_COLS columns.
+RefineOut gmres_refine(hipStream_t s, const LowFactor &f, int64_t k, const double *B, int64_t ldb, double *X,
+                       int64_t ldx) {
+    RefineOut out;
+    out.info = f.info;
+    const int64_t n = f.n;
+    const int kg = (int)std::min<int64_t>(k, GM_COLS);
+    const int64_t nk = n * kg, nchunk = cdiv(n, GM_CHUNK);
+    Scratch sV((size_t)(GM_RESTART + 1) * nk * sizeof(double), s), sW((size_t)nk * sizeof(double), s),
+        sR((size_t)nk * sizeof(double), s), sR32((size_t)nk * sizeof(float), s),
+        sP((size_t)(GM_RESTART + 1) * nchunk * kg * sizeof(double), s),
+        sD((size_t)(2 * (GM_RESTART + 1) + 1) * kg * sizeof(double), s),
+        sN((size_t)(3 * kg + 8) * sizeof(double), s);
+    GmresBuffers g{sV.as<double>(), sW.as<double>(), sR.as<double>(), sP.as<double>(), sD.as<double>(),
+                   sR32.as<float>()};
+    const double crit = std::sqrt((double)n) * EPS64;
+    bool all = true;
+    double eta = 0.0;
+    int inner = 0, outer = 0;
+    for (int64_t c0 = 0; c0 < k; c0 += GM_COLS) {
+        const int kc = (int)std::min<int64_t>(GM_COLS, k - c0);
+        bool conv = false;
+        double e = 0.0;
+        outer = std::max(outer, gmres_group(s, f, kc, B + c0, ldb, X + c0, ldx, g, sN.as<double>(), crit, &e, &conv,
+                                            &inner));
+        all = all && conv;
+        if (!(e <= eta)) eta = e;
+    }
+    out.iterations = inner;
+    out.eta = eta;
+    out.converged = all;
+    return out;
+}
 }  // namespace
 
 namespace {
@@ -331,6 +336,16 @@ void low_factor_bind(LowFactor &f, void *mem) {
     f.WU = w ? reinterpret_cast<float *>(p + a32 + 256 + ints + w) : nullptr;
 }
 
+// LMX_REFINE=gmres refines FP32 factors with GMRES-IR too (BF16 factors
+// always use it; read once per process).
+bool gmres_requested() {
+    static const bool on = [] {
+        const char *e = std::getenv("LMX_REFINE");
+        return e != nullptr && std::strcmp(e, "gmres") == 0;
+    }();
+    return on;
+}
+
 // LMX_LU_PIVOTING=tournament selects tournament pivoting for the FP32 LU
 // factor (read once per process).
 bool tournament_pivoting() {
@@ -343,12 +358,13 @@ bool tournament_pivoting() {
 
 void low_factor(hipStream_t s, LowFactor &f) {
     const int64_t n = f.n;
-    launch1d(s, n * n, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, f.A, f.lda, f.A32, n, n, n); });
-    LMX_LAUNCH_CHECK();
+    d2f(s, f.A, f.lda, f.A32, n, n, n);
+    const bool bf16 = f.precision == LMX_PREC_BF16;
     if (f.spd)
-        potrf<float>(s, n, f.A32, n, f.dinfo);
+        potrf<float>(s, n, f.A32, n, f.dinfo, bf16);
     else
-        getrf_impl<float>(s, n, f.A32, n, f.ipiv, f.perm, f.dinfo, tournament_pivoting());
+        getrf_impl<float>(s, n, f.A32, n, f.ipiv, f.perm, f.dinfo, (tournament_pivoting() || f.tournament) && !bf16,
+                          false, bf16);
     // inverses of the factor's diagonal blocks for the refinement solves (a
     // failed factor leaves them unused)
     if (f.WL) {
@@ -360,10 +376,8 @@ void low_factor(hipStream_t s, LowFactor &f) {
             tri_inverse_build<float>(s, false, false, false, n, f.A32, n, f.WU);
         }
     }
-    hipLaunchKernelGGL(zero_d_kernel, dim3(1), dim3(256), 0, s, f.anorm, 1);
-    hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 4), 1024)), dim3(256), 0, s, f.A,
-                       f.lda, n, f.anorm);
-    LMX_LAUNCH_CHECK();
+    zero_d(s, f.anorm, 1);
+    norm_inf_acc(s, f.A, f.lda, n, n, f.anorm);
     int32_t hinfo = 0;
     LMX_HIP(hipMemcpyAsync(&hinfo, f.dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     LMX_HIP(hipStreamSynchronize(s));
@@ -388,27 +402,28 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
     const int64_t lda = f.lda;
     const int K = (int)k;
     const double crit = std::sqrt((double)n) * EPS64;
-    if (f.info == 0) {
-        hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
-        LMX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(copy_d_kernel, dim3(1), dim3(256), 0, s, f.anorm, 1, norms, 1, 1, 1);
+    if (f.info == 0 && (f.precision == LMX_PREC_BF16 || gmres_requested())) {
+        RefineOut g = gmres_refine(s, f, k, B, ldb, X, ldx);
+        out.iterations = g.iterations;
+        out.eta = g.eta;
+        out.converged = g.converged;
+    } else if (f.info == 0) {
+        zero_d(s, norms, 3 * K + 8);
+        copy_d(s, f.anorm, 1, norms, 1, 1, 1);
         colmax(s, B, ldb, n, K, norms + 1);
-        LMX_LAUNCH_CHECK();
         // R = B
-        launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, R, k, n, k); });
+        copy_d(s, B, ldb, R, k, n, k);
         double prev = INFINITY;
         for (int it = 0; it < ITMAX; ++it) {
-            launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(d2f_kernel, g, dim3(256), 0, s, R, k, R32, k, n, k); });
+            d2f(s, R, k, R32, k, n, k);
             if (f.spd)
                 potrs<float>(s, n, k, f.A32, n, R32, k, f.WL, f.WU);
             else
                 getrs_perm<float>(s, n, k, f.A32, n, f.perm, R32, k, f.WL, f.WU);
-            launch1d(s, (int64_t)nk, [&](dim3 g) {
-                hipLaunchKernelGGL(f2d_acc_kernel, g, dim3(256), 0, s, R32, k, X, ldx, n, k, it == 0 ? 1 : 0);
-            });
+            f2d_acc(s, R32, k, X, ldx, n, k, it == 0);
             // R = B - A X
             residual(s, n, k, A, lda, X, ldx, B, ldb, R, k);
-            hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(2 * K + 2, 256)), dim3(256), 0, s, norms + 1 + K, 2 * K + 2);
+            zero_d(s, norms + 1 + K, 2 * K + 2);
             colmax(s, X, ldx, n, K, norms + 1 + K);
             colmax(s, R, k, n, K, norms + 1 + 2 * K);
             hipLaunchKernelGGL(eta_kernel, dim3(1), dim3(64), 0, s, norms, K, crit);
@@ -433,8 +448,8 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
         Scratch sA64(nn * sizeof(double), s);
         double *A64 = sA64.as<double>();
         int32_t *ipiv = sI.as<int32_t>(), *perm = ipiv + n, *dinfo = perm + n;
-        launch1d(s, (int64_t)nn, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, A, lda, A64, n, n, n); });
-        launch1d(s, (int64_t)nk, [&](dim3 g) { hipLaunchKernelGGL(copy_d_kernel, g, dim3(256), 0, s, B, ldb, X, ldx, n, k); });
+        copy_d(s, A, lda, A64, n, n, n);
+        copy_d(s, B, ldb, X, ldx, n, k);
         if (f.spd) {
             potrf<double>(s, n, A64, n, dinfo);
             potrs<double>(s, n, k, A64, n, X, ldx);
@@ -444,9 +459,8 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
         }
         // report the backward error of the FP64 solve
         residual(s, n, k, A, lda, X, ldx, B, ldb, R, k);
-        hipLaunchKernelGGL(zero_d_kernel, dim3((unsigned)cdiv(3 * K + 8, 256)), dim3(256), 0, s, norms, 3 * K + 8);
-        hipLaunchKernelGGL(norm_inf_rows_kernel, dim3((unsigned)std::min<int64_t>(cdiv(n, 4), 1024)), dim3(256), 0, s,
-                           A, lda, n, norms);
+        zero_d(s, norms, 3 * K + 8);
+        norm_inf_acc(s, A, lda, n, n, norms);
         colmax(s, B, ldb, n, K, norms + 1);
         colmax(s, X, ldx, n, K, norms + 1 + K);
         colmax(s, R, k, n, K, norms + 1 + 2 * K);
@@ -464,11 +478,14 @@ RefineOut low_refine(hipStream_t s, const LowFactor &f, int64_t k, const double 
     return out;
 }
 
-// precision: LMX_PREC_F32 (factor in FP32).
+// precision: LMX_PREC_F32 (factor in FP32, plain refinement) or
+// LMX_PREC_BF16 (FP32 storage, trailing updates on BF16 tiles with split
+// operands, GMRES-IR).
 RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_t k, const double *A,
                        int64_t lda, const double *B, int64_t ldb, double *X, int64_t ldx) {
     RefineOut out;
-    if (precision != LMX_PREC_F32) fail_arg("mixed solve: factor precision must be LMX_PREC_F32");
+    if (precision != LMX_PREC_F32 && precision != LMX_PREC_BF16)
+        fail_arg("mixed solve: factor precision must be LMX_PREC_F32 or LMX_PREC_BF16");
     if (n == 0 || k == 0) {
         out.converged = true;
         return out;
@@ -476,6 +493,7 @@ RefineOut refine_solve(hipStream_t s, bool spd, int precision, int64_t n, int64_
     TimedRegion trall("refine", s);
     Scratch mem(low_factor_bytes(n), s);
     LowFactor f;
+    f.precision = precision;
     f.spd = spd;
     f.n = n;
     f.A = A;

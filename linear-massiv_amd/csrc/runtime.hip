@@ -13,7 +13,8 @@ Context *g_ctx[kMaxDevices] = {};
 std::mutex g_ctx_mu;
 
 const char *kFamilies[] = {"gemm", "sgemm", "getrf_panel", "trsm", "laswp", "potrf_diag",
-                           "syrk", "getrs", "refine", "getrf", "potrf", "batch", "gemm64", "sgemm64", "ssyrk"};
+                           "syrk", "getrs", "refine", "getrf", "potrf", "batch", "gemm64", "sgemm64", "ssyrk",
+                           "bgemm", "bgemm3", "gemv"};
 constexpr int kNumFamilies = sizeof(kFamilies) / sizeof(kFamilies[0]);
 struct EventPair {
     hipEvent_t a, b;
@@ -161,6 +162,9 @@ int lmx_init(int device) {
 int lmx_finalize(void) {
     return guarded([&] {
         std::lock_guard<std::mutex> g(g_ctx_mu);
+        bool any = false;
+        for (int d = 0; d < kMaxDevices; ++d) any = any || g_ctx[d] != nullptr;
+        if (!any) return LMX_OK;  // nothing initialised: no HIP call at all
         int cur = 0;
         LMX_HIP(hipGetDevice(&cur));
         for (int d = 0; d < kMaxDevices; ++d) {

@@ -61,6 +61,8 @@ struct Request {
     bool keep = false;
     int64_t need = 0;
     bool small = false;
+    // a thread is blocked in lmx_server_wait on it (only that thread collects it)
+    bool claimed = false;
     // outcome
     bool done = false;
     int rc = 0;
@@ -193,6 +195,19 @@ void run_one(lmx_server *sv, Request *r, Factor *reuse, hipStream_t s) {
             }
             hip_free_async(mem, s);
         } else {
+            // A panel whose workgroups could not all be resident in time
+            // (info < 0: a factorisation beside many concurrent requests)
+            // is redone once with tournament pivoting, whose panels never
+            // wait on each other's workgroups.
+            auto factor_refine = [&](LowFactor &lf) {
+                low_factor(s, lf);
+                if (lf.info < 0 && !lf.spd) {
+                    LMX_HIP(hipStreamSynchronize(s));
+                    lf.tournament = true;
+                    low_factor(s, lf);
+                }
+                return low_refine(s, lf, k, dB, k, dX, k);
+            };
             std::unique_ptr<Factor> f(new Factor());
             f->kind = r->kind;
             f->n = n;
@@ -206,8 +221,7 @@ void run_one(lmx_server *sv, Request *r, Factor *reuse, hipStream_t s) {
             low_factor_bind(f->f, f->mem);
             try {
                 copy_in(f->A64, n, r->A, r->lda, n, n, s);
-                low_factor(s, f->f);
-                o = low_refine(s, f->f, k, dB, k, dX, k);
+                o = factor_refine(f->f);
             } catch (...) {
                 hip_free_async(f->A64, s);
                 hip_free_async(f->mem, s);
@@ -534,6 +548,7 @@ int lmx_server_poll(lmx_server *sv, int64_t id, lmx_request_result *res) {
         auto it = sv->requests.find(id);
         if (it == sv->requests.end()) fail_arg("unknown request id");
         Request *r = it->second;
+        if (r->claimed) return 1;  // another thread waits for it and will collect it
         fill_result(r, res);
         if (!r->done) return 1;
         sv->requests.erase(it);
@@ -549,6 +564,8 @@ int lmx_server_wait(lmx_server *sv, int64_t id, lmx_request_result *res) {
         auto it = sv->requests.find(id);
         if (it == sv->requests.end()) fail_arg("unknown request id");
         Request *r = it->second;
+        if (r->claimed) fail_arg("another thread is already waiting for this request");
+        r->claimed = true;  // from here on only this thread erases and deletes r
         sv->cv_done.wait(lk, [&] { return r->done; });
         fill_result(r, res);
         sv->requests.erase(id);

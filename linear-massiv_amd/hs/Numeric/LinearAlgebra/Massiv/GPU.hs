@@ -1,5 +1,6 @@
 {-# LANGUAGE AllowAmbiguousTypes #-}
 {-# LANGUAGE DataKinds #-}
+{-# LANGUAGE KindSignatures #-}
 {-# LANGUAGE ForeignFunctionInterface #-}
 {-# LANGUAGE ScopedTypeVariables #-}
 {-# LANGUAGE TypeApplications #-}
@@ -27,6 +28,18 @@
 --   * 'choleskySolveP' Solve/Cholesky.hs:241-273; 'choleskySolve' :229-235
 --   * 'forwardSub', 'backSub', 'forwardSubUnit', 'backSubUnit'
 --                      Solve/Triangular.hs:90-249
+--   * 'det'            Solve/LU.hs:534-558
+--   * 'inverseIteration' Eigen/Power.hs:113-135
+--   * 'matvecP', 'matvec', 'gemv'  BLAS/Level2.hs:92-144
+--   * 'norm1', 'normInf', 'normFrob', 'vnorm1', 'vnorm2', 'vnormInf'
+--                      Norms.hs:72-148
+--   * 'matMulAtAP'     BLAS/Level3.hs:353-373
+--
+-- New surface (no reference counterpart): resident factors ('luFactor',
+-- 'luSolveWith', 'luSolveManyWith', 'choleskyFactor', 'choleskySolveWith',
+-- at FP64 or a refined FP32 / BF16 precision), the refined solvers
+-- ('luSolveMixed', 'choleskySolveMixed', 'luSolveMixedWith',
+-- 'choleskySolveMixedWith') and the BF16 matrix product ('matMulBF16').
 --
 -- Error behaviour follows the reference: a singular or non-SPD input still
 -- returns a result (Inf/NaN where the arithmetic produces them, LAPACK info
@@ -54,19 +67,46 @@ module Numeric.LinearAlgebra.Massiv.GPU
   , backSubUnit
   , luSolveMixed
   , choleskySolveMixed
+  , luSolveMixedWith
+  , choleskySolveMixedWith
+  , Precision (..)
+  , det
+  , inverseIteration
+  , matvecP
+  , matvec
+  , gemv
+  , norm1
+  , normInf
+  , normFrob
+  , vnorm1
+  , vnorm2
+  , vnormInf
+  , matMulAtAP
+  , matMulBF16
+  , Factor
+  , luFactor
+  , choleskyFactor
+  , luSolveWith
+  , luSolveManyWith
+  , choleskySolveWith
   , GpuError (..)
   ) where
 
 import Control.Exception (Exception, throwIO)
-import Control.Monad (when)
+import Control.Monad (void, when)
 import Data.Int (Int32, Int64)
+import Data.Word (Word16)
+import Foreign.Concurrent (newForeignPtr)
+import Foreign.ForeignPtr (ForeignPtr, withForeignPtr)
+import Foreign.Marshal.Alloc (alloca)
+import Foreign.Storable (peek)
 import qualified Data.Massiv.Array as M
 import Foreign.C.String (CString, peekCString)
 import Foreign.C.Types (CChar (..), CDouble (..), CInt (..))
 import Foreign.Marshal.Alloc (allocaBytes)
 import Foreign.Marshal.Array (allocaArray, peekArray)
 import Foreign.Ptr (Ptr, castPtr)
-import GHC.TypeNats (KnownNat)
+import GHC.TypeNats (KnownNat, Nat)
 import System.IO.Unsafe (unsafePerformIO)
 
 import Numeric.LinearAlgebra.Massiv.Internal (dimVal)
@@ -96,6 +136,32 @@ foreign import ccall safe "lmx.h lmx_dgesv_mixed"
 foreign import ccall safe "lmx.h lmx_dposv_mixed"
   c_dposv_mixed :: CInt -> Int64 -> Int64 -> Ptr Double -> Int64 -> Ptr Double -> Int64
                 -> Ptr Double -> Int64 -> Ptr () -> IO CInt
+foreign import ccall safe "lmx.h lmx_dlu"
+  c_dlu :: Int64 -> Ptr Double -> Int64 -> Ptr Int32 -> IO CInt
+foreign import ccall safe "lmx.h lmx_slu"
+  c_slu :: Int64 -> Ptr Float -> Int64 -> Ptr Int32 -> IO CInt
+foreign import ccall safe "lmx.h lmx_ddet"
+  c_ddet :: Int64 -> Ptr Double -> Int64 -> Ptr Double -> IO CInt
+foreign import ccall safe "lmx.h lmx_dinverse_iteration"
+  c_dinverse_iteration :: Int64 -> Ptr Double -> Int64 -> CDouble -> Ptr Double -> Int32 -> CDouble
+                       -> Ptr Double -> Ptr Double -> Ptr Int32 -> IO CInt
+foreign import ccall safe "lmx.h lmx_dgemv"
+  c_dgemv :: Int64 -> Int64 -> CDouble -> Ptr Double -> Int64 -> Ptr Double -> CDouble -> Ptr Double
+          -> IO CInt
+foreign import ccall safe "lmx.h lmx_dnorm"
+  c_dnorm :: CChar -> Int64 -> Int64 -> Ptr Double -> Int64 -> Ptr Double -> IO CInt
+foreign import ccall safe "lmx.h lmx_dsyrk"
+  c_dsyrk :: CChar -> Int64 -> Int64 -> CDouble -> Ptr Double -> Int64 -> CDouble -> Ptr Double -> Int64
+          -> IO CInt
+foreign import ccall safe "lmx.h lmx_gemm_bf16"
+  c_gemm_bf16 :: CChar -> CChar -> Int64 -> Int64 -> Int64 -> Float -> Ptr Word16 -> Int64 -> Ptr Word16
+              -> Int64 -> Float -> Ptr Float -> Int64 -> IO CInt
+foreign import ccall safe "lmx.h lmx_factor_create"
+  c_factor_create :: Ptr (Ptr ()) -> CInt -> CInt -> Int64 -> Ptr Double -> Int64 -> IO CInt
+foreign import ccall safe "lmx.h lmx_factor_solve"
+  c_factor_solve :: Ptr () -> Int64 -> Ptr Double -> Int64 -> Ptr Double -> Int64 -> Ptr () -> IO CInt
+foreign import ccall safe "lmx.h lmx_factor_release"
+  c_factor_release :: Ptr () -> IO CInt
 foreign import ccall unsafe "lmx.h lmx_last_error"
   c_last_error :: IO CString
 
@@ -171,43 +237,32 @@ gemm alpha (MkMatrix a) (MkMatrix b) beta (MkMatrix c0) = unsafePerformIO $ do
 -- ------------------------------------------------------------------ LU
 
 -- | Packed LU and the permutation vector (row i of PA is row piv!i of A),
--- the output format of lu (Solve/LU.hs:121-167).  The device returns the
--- LAPACK swap list; the permutation is its composition.
+-- the output of lu (Solve/LU.hs:121-167), with lu's own pivot rule: the
+-- last row of largest |a_ik| wins a tie (maximumBy, :128-135).  lmx_dlu
+-- returns the permutation vector itself.
 lu :: forall n. KnownNat n
    => Matrix n n M.P Double -> (Matrix n n M.P Double, M.Array M.P M.Ix1 Int)
 lu (MkMatrix a) = unsafePerformIO $ do
   let nn = dimVal @n
-  allocaArray (max 1 nn) $ \pipiv -> do
-    f <- inPlace2 a $ \pa -> () <$ (check =<< c_dgetrf (fi nn) pa (fi (max 1 nn)) pipiv)
-    swaps <- map fromIntegral <$> peekArray nn pipiv
-    pure (MkMatrix f, M.fromList M.Seq (swapsToPerm nn swaps))
+  allocaArray (max 1 nn) $ \pperm -> do
+    f <- inPlace2 a $ \pa -> () <$ (check =<< c_dlu (fi nn) pa (fi (max 1 nn)) pperm)
+    perm <- map fromIntegral <$> peekArray nn pperm
+    pure (MkMatrix f, M.fromList M.Seq perm)
 {-# NOINLINE lu #-}
 
 -- | lu at e = Float (the reference's lu is generic over Fractional e,
--- Solve/LU.hs:121): the FP32 factorisation, with the same pivot rules.
+-- Solve/LU.hs:121): the FP32 factorisation, with the same pivot rule.
 luFloat :: forall n. KnownNat n
         => Matrix n n M.P Float -> (Matrix n n M.P Float, M.Array M.P M.Ix1 Int)
 luFloat (MkMatrix a) = unsafePerformIO $ do
   let nn = dimVal @n
-  allocaArray (max 1 nn) $ \pipiv -> do
+  allocaArray (max 1 nn) $ \pperm -> do
     ma <- M.thawS (M.computeAs M.S (M.delay a) :: M.Array M.S M.Ix2 Float)
-    M.withPtr ma $ \pa -> () <$ (check =<< c_sgetrf (fi nn) pa (fi (max 1 nn)) pipiv)
+    M.withPtr ma $ \pa -> () <$ (check =<< c_slu (fi nn) pa (fi (max 1 nn)) pperm)
     f <- M.computeAs M.P . M.delay <$> M.freezeS ma
-    swaps <- map fromIntegral <$> peekArray nn pipiv
-    pure (MkMatrix f, M.fromList M.Seq (swapsToPerm nn swaps))
+    perm <- map fromIntegral <$> peekArray nn pperm
+    pure (MkMatrix f, M.fromList M.Seq perm)
 {-# NOINLINE luFloat #-}
-
-swapsToPerm :: Int -> [Int] -> [Int]
-swapsToPerm n swaps = M.toList (M.computeAs M.P (go (M.fromList M.Seq [0 .. n - 1]) (zip [0 ..] swaps)))
-  where
-    go :: M.Array M.P M.Ix1 Int -> [(Int, Int)] -> M.Array M.D M.Ix1 Int
-    go p [] = M.delay p
-    go p ((k, q) : rest)
-      | k == q = go p rest
-      | otherwise =
-          let pk = p M.! k; pq = p M.! q
-              p' = M.computeAs M.P (M.imap (\i x -> if i == k then pq else if i == q then pk else x) p)
-          in go p' rest
 
 -- | x with A x = b (replaces luSolveP, Solve/LU.hs:265-304).
 luSolveP :: forall n. KnownNat n
@@ -300,17 +355,196 @@ mixed :: forall n. KnownNat n
       => (CInt -> Int64 -> Int64 -> Ptr Double -> Int64 -> Ptr Double -> Int64
           -> Ptr Double -> Int64 -> Ptr () -> IO CInt)
       -> Matrix n n M.P Double -> Vector n M.P Double -> Vector n M.P Double
-mixed call (MkMatrix a) (MkVector b) = unsafePerformIO $ do
+mixed call = mixedAt @n call F32
+
+-- | The factor precision of the refined solvers and resident factors:
+-- FP64, FP32, or BF16 tiles with FP32 storage (LMX_PREC_*).
+data Precision = F64 | F32 | BF16 deriving (Eq, Show)
+
+precCode :: Precision -> CInt
+precCode F64 = 0
+precCode F32 = 1
+precCode BF16 = 2
+
+-- | luSolveMixed at a chosen factor precision (F32 or BF16).
+luSolveMixedWith :: forall n. KnownNat n
+                 => Precision -> Matrix n n M.P Double -> Vector n M.P Double -> Vector n M.P Double
+luSolveMixedWith = mixedAt @n c_dgesv_mixed
+
+choleskySolveMixedWith :: forall n. KnownNat n
+                       => Precision -> Matrix n n M.P Double -> Vector n M.P Double -> Vector n M.P Double
+choleskySolveMixedWith = mixedAt @n c_dposv_mixed
+
+mixedAt :: forall n. KnownNat n
+        => (CInt -> Int64 -> Int64 -> Ptr Double -> Int64 -> Ptr Double -> Int64
+            -> Ptr Double -> Int64 -> Ptr () -> IO CInt)
+        -> Precision -> Matrix n n M.P Double -> Vector n M.P Double -> Vector n M.P Double
+mixedAt call prec (MkMatrix a) (MkVector b) = unsafePerformIO $ do
   let nn = dimVal @n
   ma <- M.thawS (toS2 a)
   mb <- M.thawS (toS1 b)
   mx <- M.thawS (toS1 b)
   allocaBytes 32 $ \info ->
     M.withPtr ma $ \pa -> M.withPtr mb $ \pb -> M.withPtr mx $ \px -> do
-      rc <- call 1 (fi nn) 1 pa (fi (max 1 nn)) pb 1 px 1 (castPtr info)
+      rc <- call (precCode prec) (fi nn) 1 pa (fi (max 1 nn)) pb 1 px 1 (castPtr info)
       when (rc < 0) (() <$ check rc)
   MkVector . fromS1 <$> M.freezeS mx
-{-# NOINLINE mixed #-}
+{-# NOINLINE mixedAt #-}
+
+-- ------------------------------------------------------------------ det, inverse iteration
+
+-- | det (Solve/LU.hs:534-558): lu's factors, the product of U's diagonal in
+-- order and the sign of the permutation.
+det :: forall n. KnownNat n => Matrix n n M.P Double -> Double
+det (MkMatrix a) = unsafePerformIO $ do
+  let nn = dimVal @n
+  ms <- M.thawS (toS2 a)
+  alloca $ \pd -> do
+    M.withPtr ms $ \pa -> () <$ (check =<< c_ddet (fi nn) pa (fi (max 1 nn)) pd)
+    realToFrac <$> (peek pd :: IO Double)
+{-# NOINLINE det #-}
+
+-- | inverseIteration (Eigen/Power.hs:113-135): A - mu I is factored once on
+-- the device; each step is a solve, a norm, a GEMV and a dot product there.
+inverseIteration :: forall n. KnownNat n
+                 => Matrix n n M.P Double -> Double -> Vector n M.P Double -> Int -> Double
+                 -> (Double, Vector n M.P Double)
+inverseIteration (MkMatrix a) mu (MkVector q0) maxIter tol = unsafePerformIO $ do
+  let nn = dimVal @n
+  ms <- M.thawS (toS2 a)
+  mq0 <- M.thawS (toS1 q0)
+  mq <- M.thawS (toS1 q0)
+  lam <- alloca $ \pl -> alloca $ \pit ->
+    M.withPtr ms $ \pa -> M.withPtr mq0 $ \pq0 -> M.withPtr mq $ \pq -> do
+      _ <- check =<< c_dinverse_iteration (fi nn) pa (fi (max 1 nn)) (realToFrac mu) pq0
+                                          (fromIntegral maxIter) (realToFrac tol) pl pq pit
+      peek pl
+  q <- fromS1 <$> M.freezeS mq
+  pure (lam, MkVector q)
+{-# NOINLINE inverseIteration #-}
+
+-- ------------------------------------------------------------------ level 2, norms, SYRK
+
+-- | gemv alpha A x beta y (BLAS/Level2.hs:92-100).  beta = 0 does not read y.
+gemv :: forall m n. (KnownNat m, KnownNat n)
+     => Double -> Matrix m n M.P Double -> Vector n M.P Double -> Double -> Vector m M.P Double
+     -> Vector m M.P Double
+gemv alpha (MkMatrix a) (MkVector x) beta (MkVector y) = unsafePerformIO $ do
+  let mm = dimVal @m; nn = dimVal @n
+  ms <- M.thawS (toS2 a)
+  mx <- M.thawS (toS1 x)
+  my <- M.thawS (toS1 y)
+  M.withPtr ms $ \pa -> M.withPtr mx $ \px -> M.withPtr my $ \py ->
+    () <$ (check =<< c_dgemv (fi mm) (fi nn) (realToFrac alpha) pa (fi (max 1 nn)) px (realToFrac beta) py)
+  MkVector . fromS1 <$> M.freezeS my
+{-# NOINLINE gemv #-}
+
+-- | matvecP (BLAS/Level2.hs:132-140) and matvec (:123-130) at P Double.
+matvecP :: forall m n. (KnownNat m, KnownNat n)
+        => Matrix m n M.P Double -> Vector n M.P Double -> Vector m M.P Double
+matvecP a x = gemv @m @n 1 a x 0 (MkVector (M.replicate M.Seq (M.Sz1 (dimVal @m)) 0))
+
+matvec :: forall m n. (KnownNat m, KnownNat n)
+       => Matrix m n M.P Double -> Vector n M.P Double -> Vector m M.P Double
+matvec = matvecP @m @n
+
+normWith :: forall m n. (KnownNat m, KnownNat n) => Char -> Matrix m n M.P Double -> Double
+normWith which (MkMatrix a) = unsafePerformIO $ do
+  let mm = dimVal @m; nn = dimVal @n
+  ms <- M.thawS (toS2 a)
+  alloca $ \pd -> do
+    M.withPtr ms $ \pa -> () <$ (check =<< c_dnorm (chr which) (fi mm) (fi nn) pa (fi (max 1 nn)) pd)
+    peek pd
+{-# NOINLINE normWith #-}
+
+-- | Norms.hs:124-130, :142-148, :110-112.
+norm1, normInf, normFrob :: forall m n. (KnownNat m, KnownNat n) => Matrix m n M.P Double -> Double
+norm1 = normWith @m @n '1'
+normInf = normWith @m @n 'I'
+normFrob = normWith @m @n 'F'
+
+asColumn :: forall n. KnownNat n => Vector n M.P Double -> Matrix n 1 M.P Double
+asColumn (MkVector x) = MkMatrix (M.resize' (M.Sz2 (dimVal @n) 1) x)
+
+-- | Norms.hs:72-74, :83-85, :94-96.
+vnorm1, vnorm2, vnormInf :: forall n. KnownNat n => Vector n M.P Double -> Double
+vnorm1 = normWith @n @1 '1' . asColumn
+vnorm2 = normWith @n @1 'F' . asColumn
+vnormInf = normWith @n @1 'I' . asColumn
+
+-- | matMulAtAP (BLAS/Level3.hs:353-373): A^T A, the whole symmetric matrix.
+matMulAtAP :: forall m n. (KnownNat m, KnownNat n) => Matrix m n M.P Double -> Matrix n n M.P Double
+matMulAtAP (MkMatrix a) = unsafePerformIO $ do
+  let mm = dimVal @m; nn = dimVal @n
+      c0 = M.replicate M.Seq (M.Sz2 nn nn) 0 :: M.Array M.P M.Ix2 Double
+  ms <- M.thawS (toS2 a)
+  c <- inPlace2 c0 $ \pc -> M.withPtr ms $ \pa ->
+    () <$ (check =<< c_dsyrk (chr 'T') (fi nn) (fi mm) 1 pa (fi (max 1 nn)) 0 pc (fi (max 1 nn)))
+  pure (MkMatrix c)
+{-# NOINLINE matMulAtAP #-}
+
+-- | A B on the BF16 matrix cores with FP32 accumulation; the operands hold
+-- BF16 bit patterns (the upper half of an IEEE single).
+matMulBF16 :: forall m k n. (KnownNat m, KnownNat k, KnownNat n)
+           => Matrix m k M.P Word16 -> Matrix k n M.P Word16 -> Matrix m n M.P Float
+matMulBF16 (MkMatrix a) (MkMatrix b) = unsafePerformIO $ do
+  let mm = dimVal @m; kk = dimVal @k; nn = dimVal @n
+  ma <- M.thawS (M.computeAs M.S (M.delay a) :: M.Array M.S M.Ix2 Word16)
+  mb <- M.thawS (M.computeAs M.S (M.delay b) :: M.Array M.S M.Ix2 Word16)
+  mc <- M.thawS (M.replicate M.Seq (M.Sz2 mm nn) 0 :: M.Array M.S M.Ix2 Float)
+  M.withPtr ma $ \pa -> M.withPtr mb $ \pb -> M.withPtr mc $ \pc ->
+    () <$ (check =<< c_gemm_bf16 (chr 'N') (chr 'N') (fi mm) (fi nn) (fi kk) 1 pa (fi (max 1 kk)) pb
+                                (fi (max 1 nn)) 0 pc (fi (max 1 nn)))
+  MkMatrix . M.computeAs M.P . M.delay <$> M.freezeS mc
+{-# NOINLINE matMulBF16 #-}
+
+-- ------------------------------------------------------------------ resident factors
+
+-- | A factor of an n x n matrix kept in HBM (lmx_factor_*); released by the
+-- garbage collector.  FP64 factors solve by substitution through the kept
+-- triangular inverses; F32 / BF16 factors refine every solve to FP64
+-- accuracy.
+newtype Factor (n :: Nat) = Factor (ForeignPtr ())
+
+mkFactor :: forall n. KnownNat n => CInt -> Precision -> Matrix n n M.P Double -> Factor n
+mkFactor kind prec (MkMatrix a) = unsafePerformIO $ do
+  let nn = dimVal @n
+  ms <- M.thawS (toS2 a)
+  h <- alloca $ \ph -> do
+    M.withPtr ms $ \pa -> () <$ (check =<< c_factor_create ph kind (precCode prec) (fi nn) pa (fi (max 1 nn)))
+    peek ph
+  Factor <$> newForeignPtr h (void (c_factor_release h))
+{-# NOINLINE mkFactor #-}
+
+-- | An LU factor (partial pivoting) kept on the device.
+luFactor :: forall n. KnownNat n => Precision -> Matrix n n M.P Double -> Factor n
+luFactor = mkFactor @n 0
+
+-- | A Cholesky factor kept on the device.
+choleskyFactor :: forall n. KnownNat n => Precision -> Matrix n n M.P Double -> Factor n
+choleskyFactor = mkFactor @n 1
+
+-- | Many right-hand sides against one resident factor.
+luSolveManyWith :: forall n k. (KnownNat n, KnownNat k)
+                => Factor n -> Matrix n k M.P Double -> Matrix n k M.P Double
+luSolveManyWith (Factor fp) (MkMatrix b) = unsafePerformIO $ do
+  let nn = dimVal @n; kk = dimVal @k
+  mb <- M.thawS (toS2 b)
+  mx <- M.thawS (toS2 b)
+  withForeignPtr fp $ \h -> allocaBytes 32 $ \info ->
+    M.withPtr mb $ \pb -> M.withPtr mx $ \px -> do
+      rc <- c_factor_solve h (fi kk) pb (fi (max 1 kk)) px (fi (max 1 kk)) (castPtr info)
+      when (rc < 0) (() <$ check rc)
+  MkMatrix . fromS2 <$> M.freezeS mx
+{-# NOINLINE luSolveManyWith #-}
+
+luSolveWith :: forall n. KnownNat n => Factor n -> Vector n M.P Double -> Vector n M.P Double
+luSolveWith f (MkVector b) =
+  let MkMatrix x = luSolveManyWith @n @1 f (MkMatrix (M.resize' (M.Sz2 (dimVal @n) 1) b))
+  in MkVector (M.resize' (M.Sz1 (dimVal @n)) x)
+
+choleskySolveWith :: forall n. KnownNat n => Factor n -> Vector n M.P Double -> Vector n M.P Double
+choleskySolveWith = luSolveWith @n
 
 fi :: Int -> Int64
 fi = fromIntegral

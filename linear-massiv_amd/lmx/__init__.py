@@ -22,6 +22,7 @@ the library is missing the import fails.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 
@@ -56,6 +57,13 @@ EXPORTED = [
     "lmx_init", "lmx_finalize", "lmx_last_error", "lmx_version",
     "lmx_dgemm", "lmx_dgemm_dev", "lmx_sgemm", "lmx_sgemm_dev",
     "lmx_dgetrf", "lmx_dgetrf_dev", "lmx_sgetrf", "lmx_sgetrf_dev", "lmx_dgetrs", "lmx_dgetrs_dev", "lmx_dgesv", "lmx_dgesv_dev",
+    "lmx_dlu", "lmx_dlu_dev", "lmx_slu", "lmx_slu_dev",
+    "lmx_gemm_bf16", "lmx_gemm_bf16_dev", "lmx_sgemm_bf16x3_dev",
+    "lmx_dgemv", "lmx_dgemv_dev", "lmx_dnorm", "lmx_dnorm_dev", "lmx_dresidual_dev", "lmx_dscaled_residual",
+    "lmx_dscaled_residual_dev", "lmx_dsyrk", "lmx_dsyrk_dev", "lmx_ddet", "lmx_ddet_dev",
+    "lmx_dinverse_iteration", "lmx_dinverse_iteration_dev",
+    "lmx_factor_create", "lmx_factor_create_dev", "lmx_factor_solve", "lmx_factor_solve_dev", "lmx_factor_query",
+    "lmx_factor_release",
     "lmx_dpotrf", "lmx_dpotrf_dev", "lmx_dpotrs", "lmx_dpotrs_dev", "lmx_dposv", "lmx_dposv_dev",
     "lmx_dtrsm", "lmx_dtrsm_dev",
     "lmx_dgesv_mixed", "lmx_dgesv_mixed_dev", "lmx_dposv_mixed", "lmx_dposv_mixed_dev",
@@ -124,6 +132,35 @@ _sigs = {
     "lmx_dgetrf_dev": [_vp, _i64, _vp, _i64, _vp, _vp],
     "lmx_sgetrf": [_i64, _vp, _i64, _vp],
     "lmx_sgetrf_dev": [_vp, _i64, _vp, _i64, _vp, _vp],
+    "lmx_gemm_bf16": [_chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
+    "lmx_gemm_bf16_dev": [_vp, _chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
+    "lmx_sgemm_bf16x3_dev": [_vp, _chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
+    "lmx_dgemv": [_i64, _i64, _dbl, _vp, _i64, _vp, _dbl, _vp],
+    "lmx_dgemv_dev": [_vp, _i64, _i64, _dbl, _vp, _i64, _vp, _dbl, _vp],
+    "lmx_dnorm": [_chr, _i64, _i64, _vp, _i64, ctypes.POINTER(_dbl)],
+    "lmx_dnorm_dev": [_vp, _chr, _i64, _i64, _vp, _i64, _vp],
+    "lmx_dresidual_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
+    "lmx_dscaled_residual": [_i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
+    "lmx_dscaled_residual_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
+    "lmx_dsyrk": [_chr, _i64, _i64, _dbl, _vp, _i64, _dbl, _vp, _i64],
+    "lmx_dsyrk_dev": [_vp, _chr, _i64, _i64, _dbl, _vp, _i64, _dbl, _vp, _i64],
+    "lmx_ddet": [_i64, _vp, _i64, ctypes.POINTER(_dbl)],
+    "lmx_ddet_dev": [_vp, _i64, _vp, _i64, _vp],
+    "lmx_dinverse_iteration": [_i64, _vp, _i64, _dbl, _vp, _i32, _dbl, ctypes.POINTER(_dbl), _vp,
+                               ctypes.POINTER(_i32)],
+    "lmx_dinverse_iteration_dev": [_vp, _i64, _vp, _i64, _dbl, _vp, _i32, _dbl, ctypes.POINTER(_dbl), _vp,
+                                   ctypes.POINTER(_i32)],
+    "lmx_factor_create": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _i64, _vp, _i64],
+    "lmx_factor_create_dev": [ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, _i64, _vp, _i64],
+    "lmx_factor_solve": [_vp, _i64, _vp, _i64, _vp, _i64, _pRI],
+    "lmx_factor_solve_dev": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _pRI],
+    "lmx_factor_query": [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(_i32),
+                         ctypes.POINTER(_i32), ctypes.POINTER(_i64)],
+    "lmx_factor_release": [_vp],
+    "lmx_dlu": [_i64, _vp, _i64, _vp],
+    "lmx_dlu_dev": [_vp, _i64, _vp, _i64, _vp, _vp],
+    "lmx_slu": [_i64, _vp, _i64, _vp],
+    "lmx_slu_dev": [_vp, _i64, _vp, _i64, _vp, _vp],
     "lmx_dgetrs": [_i64, _i64, _vp, _i64, _vp, _vp, _i64],
     "lmx_dgetrs_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64],
     "lmx_dgesv": [_i64, _i64, _vp, _i64, _vp, _vp, _i64],
@@ -165,6 +202,20 @@ for _name, _args in _sigs.items():
     _f.restype = ctypes.c_int
 _lib.lmx_last_error.argtypes = []
 _lib.lmx_last_error.restype = ctypes.c_char_p
+
+
+def _finalize_at_exit():
+    # The library's contexts (a stream, a memory pool that keeps freed
+    # memory, staging buffers) are released here, while the HIP runtime and
+    # torch are still whole; left to process exit they would be torn down
+    # after (or while) the runtime and any profiler attached to it unload.
+    try:
+        _lib.lmx_finalize()
+    except Exception:
+        pass
+
+
+atexit.register(_finalize_at_exit)  # registered after torch's: runs before them
 
 
 def missing_symbols():
@@ -240,6 +291,31 @@ def sgemm(A, B, C=None, alpha=1.0, beta=0.0, transa="N", transb="N"):
     C = np.zeros((m, n), np.float32) if C is None else np.array(C, np.float32, order="C", copy=True)
     _check(_lib.lmx_sgemm(transa.encode(), transb.encode(), m, n, k, alpha, _ptr(A), max(A.shape[1], 1),
                           _ptr(B), max(B.shape[1], 1), beta, _ptr(C), max(n, 1)))
+    return C
+
+
+def to_bf16_bits(x):
+    """float32 -> BF16 bit patterns (uint16), rounding to nearest even."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+def from_bf16_bits(h):
+    """BF16 bit patterns (uint16) -> float32."""
+    return (np.ascontiguousarray(h, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def gemm_bf16(A, B, C=None, alpha=1.0, beta=0.0, transa="N", transb="N"):
+    """C (float32) = alpha op(A) op(B) + beta C with A, B given as BF16 bit
+    patterns (uint16 arrays), on BF16 matrix cores with FP32 accumulation."""
+    A = np.ascontiguousarray(A, np.uint16)
+    B = np.ascontiguousarray(B, np.uint16)
+    m = A.shape[1] if transa == "T" else A.shape[0]
+    k = A.shape[0] if transa == "T" else A.shape[1]
+    n = B.shape[0] if transb == "T" else B.shape[1]
+    C = np.zeros((m, n), np.float32) if C is None else np.array(C, np.float32, order="C", copy=True)
+    _check(_lib.lmx_gemm_bf16(transa.encode(), transb.encode(), m, n, k, alpha, _ptr(A), max(A.shape[1], 1),
+                              _ptr(B), max(B.shape[1], 1), beta, _ptr(C), max(n, 1)))
     return C
 
 
@@ -366,9 +442,23 @@ def gemm(alpha, a, b, beta, c):
 
 
 def lu(a):
-    """lu (Solve/LU.hs:121-167) -> (packed LU, permutation vector)."""
-    LU, ipiv, _ = getrf(a)
-    return LU, swaps_to_perm(ipiv)
+    """lu (Solve/LU.hs:121-167) -> (packed LU, permutation vector), with lu's
+    own pivot rule: the last row of largest |a_ik| wins a tie (maximumBy,
+    :128-135); the pivots of luSolveP (getrf) take the first."""
+    A = _f64(a)
+    n = A.shape[0]
+    perm = np.zeros(max(n, 1), np.int32)
+    _check(_lib.lmx_dlu(n, _ptr(A), max(n, 1), _ptr(perm)), allow_info=True)
+    return A, perm[:n].astype(np.int64)
+
+
+def lu_float(a):
+    """lu at e = Float: -> (packed LU as float32, permutation vector)."""
+    A = np.array(a, np.float32, order="C", copy=True)
+    n = A.shape[0]
+    perm = np.zeros(max(n, 1), np.int32)
+    _check(_lib.lmx_slu(n, _ptr(A), max(n, 1), _ptr(perm)), allow_info=True)
+    return A, perm[:n].astype(np.int64)
 
 
 def luSolveP(a, b):
@@ -411,6 +501,193 @@ def backSubUnit(u, b):
     return trsm(u, b, "U", "N", "U")      # Solve/Triangular.hs:235-249
 
 
+# ---------------------------------------------------------------- resident factors
+
+FACTOR_LU, FACTOR_CHOLESKY = 0, 1
+
+
+class Factor:
+    """A factor resident in HBM (lmx_factor_*): factor once, solve many
+    times.  precision PREC_F64 keeps the FP64 factor; PREC_F32 / PREC_BF16
+    keep the low-precision factor and refine every solve to FP64 accuracy.
+    `a` is a numpy matrix (host) or a CUDA float64 tensor (device)."""
+
+    def __init__(self, a, kind=FACTOR_LU, precision=PREC_F64, stream=None):
+        h = _vp()
+        try:
+            import torch
+            dev = isinstance(a, torch.Tensor)
+        except ImportError:
+            dev = False
+        if dev:
+            n = _square(a, "A", "f64")
+            rc = _lib.lmx_factor_create_dev(ctypes.byref(h), _stream(stream), kind, precision, n, _tp(a), _ld(a))
+        else:
+            A = _f64(a, copy=False)
+            n = A.shape[0]
+            rc = _lib.lmx_factor_create(ctypes.byref(h), kind, precision, n, _ptr(A), max(n, 1))
+        self.info = _check(rc, allow_info=True)
+        self._h = h
+        self.n, self.kind, self.precision = n, kind, precision
+
+    def solve(self, b):
+        """Host solve: b is a vector or an n x nrhs block -> (x, RefineInfo)."""
+        B, vec = _as_block(b)
+        if B.shape[0] != self.n:
+            raise ValueError(f"b has {B.shape[0]} rows, the factor is of order {self.n}")
+        k = B.shape[1]
+        X = np.zeros_like(B)
+        ri = RefineInfo()
+        _check(_lib.lmx_factor_solve(self._h, k, _ptr(B), max(k, 1), _ptr(X), max(k, 1), ctypes.byref(ri)),
+               allow_info=True)
+        return (X[:, 0] if vec else X), ri
+
+    def solve_dev(self, B, X, stream=None):
+        _dev_mat(B, "B", "f64", rows=self.n)
+        _dev_mat(X, "X", "f64", rows=self.n, cols=B.shape[1])
+        ri = RefineInfo()
+        _check(_lib.lmx_factor_solve_dev(self._h, _stream(stream), B.shape[1], _tp(B), _ld(B), _tp(X), _ld(X),
+                                         ctypes.byref(ri)), allow_info=True)
+        return ri
+
+    def query(self):
+        info, n, kind, prec, nbytes = _i32(), _i64(), _i32(), _i32(), _i64()
+        _check(_lib.lmx_factor_query(self._h, ctypes.byref(info), ctypes.byref(n), ctypes.byref(kind),
+                                     ctypes.byref(prec), ctypes.byref(nbytes)))
+        return {"info": info.value, "n": n.value, "kind": kind.value, "precision": prec.value,
+                "bytes": nbytes.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(_lib.lmx_factor_release(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def luFactor(a, precision=PREC_F64):
+    """An LU factor kept on the device (no reference counterpart: lu,
+    Solve/LU.hs:121, returns a factor nothing reuses)."""
+    return Factor(a, FACTOR_LU, precision)
+
+
+def luSolveWith(f, b):
+    """x with A x = b for the factored A (a vector or an n x nrhs block)."""
+    return f.solve(b)[0]
+
+
+def choleskyFactor(a, precision=PREC_F64):
+    return Factor(a, FACTOR_CHOLESKY, precision)
+
+
+choleskySolveWith = luSolveWith
+
+
+# ------------------------------------- level 2, norms, SYRK, det, inverse iteration
+
+def gemv(alpha, a, x, beta, y):
+    """gemv alpha a x beta y (BLAS/Level2.hs:92-100) -> alpha a x + beta y
+    (beta = 0 does not read y)."""
+    A = _f64(a, copy=False)
+    x = _f64(x)
+    y = _f64(y)
+    m, n = A.shape
+    _check(_lib.lmx_dgemv(m, n, alpha, _ptr(A), max(n, 1), _ptr(x), beta, _ptr(y)))
+    return y
+
+
+def matvec(a, x):
+    """matvec / matvecP (BLAS/Level2.hs:123-144)."""
+    A = _f64(a, copy=False)
+    return gemv(1.0, A, x, 0.0, np.zeros(A.shape[0]))
+
+
+matvecP = matvec
+
+
+def norm(a, which):
+    """A matrix norm: '1' (norm1), 'I' (normInf), 'F' (normFrob), 'M' (max |a_ij|)."""
+    A = _f64(np.atleast_2d(a), copy=False)
+    m, n = A.shape
+    out = _dbl(0.0)
+    _check(_lib.lmx_dnorm(which.encode(), m, n, _ptr(A), max(n, 1), ctypes.byref(out)))
+    return out.value
+
+
+def norm1(a):
+    return norm(a, "1")          # Norms.hs:124-130
+
+
+def normInf(a):
+    return norm(a, "I")          # Norms.hs:142-148
+
+
+def normFrob(a):
+    return norm(a, "F")          # Norms.hs:110-112
+
+
+def vnorm1(x):
+    return norm(np.reshape(x, (-1, 1)), "1")     # Norms.hs:72-74
+
+
+def vnorm2(x):
+    return norm(np.reshape(x, (-1, 1)), "F")     # Norms.hs:83-85
+
+
+def vnormInf(x):
+    return norm(np.reshape(x, (-1, 1)), "I")     # Norms.hs:94-96
+
+
+def scaled_residual_linear(a, x, b):
+    """scaledResidualLinear (test/Test/Residuals.hs:69-78), per column of x / b."""
+    A = _f64(a, copy=False)
+    X, vec = _as_block(x)
+    B, _ = _as_block(b)
+    n, k = X.shape
+    eta = np.zeros(max(k, 1))
+    _check(_lib.lmx_dscaled_residual(n, k, _ptr(A), max(n, 1), _ptr(X), max(k, 1), _ptr(B), max(k, 1), _ptr(eta)))
+    return float(eta[0]) if vec else eta[:k]
+
+
+def syrk(a, trans="N", alpha=1.0, beta=0.0, c=None):
+    """C = alpha op(a) op(a)^T + beta C, the whole symmetric matrix."""
+    A = _f64(a, copy=False)
+    n, k = (A.shape[1], A.shape[0]) if trans == "T" else A.shape
+    C = np.zeros((n, n)) if c is None else _f64(c)
+    _check(_lib.lmx_dsyrk(trans.encode(), n, k, alpha, _ptr(A), max(A.shape[1], 1), beta, _ptr(C), max(n, 1)))
+    return C
+
+
+def matMulAtAP(a):
+    """matMulAtAP (BLAS/Level3.hs:353-373): A^T A."""
+    return syrk(a, "T")
+
+
+def det(a):
+    """det (Solve/LU.hs:534-558)."""
+    A = _f64(a, copy=False)
+    out = _dbl(0.0)
+    _check(_lib.lmx_ddet(A.shape[0], _ptr(A), max(A.shape[0], 1), ctypes.byref(out)))
+    return out.value
+
+
+def inverseIteration(a, mu, q0, max_iter, tol):
+    """inverseIteration (Eigen/Power.hs:113-135) -> (lambda, q, steps)."""
+    A = _f64(a, copy=False)
+    q0 = _f64(q0)
+    n = A.shape[0]
+    q = np.zeros(max(n, 1))
+    lam = _dbl(0.0)
+    it = _i32(0)
+    _check(_lib.lmx_dinverse_iteration(n, _ptr(A), max(n, 1), mu, _ptr(q0), max_iter, tol, ctypes.byref(lam),
+                                       _ptr(q), ctypes.byref(it)))
+    return lam.value, q[:n], it.value
+
+
 # ---------------------------------------------------------------- device API (torch)
 
 def _tp(t):
@@ -424,68 +701,246 @@ def _stream(stream):
     return ctypes.c_void_p(stream)
 
 
-def dgemm_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", stream=None):
+def _torch_dtype(name):
+    import torch
+    return {"f64": torch.float64, "f32": torch.float32, "i32": torch.int32, "bf16": torch.bfloat16}[name]
+
+
+def _dev_mat(t, name, dtype, rows=None, cols=None):
+    """Checks a device matrix argument before its pointer reaches the
+    library: a CUDA tensor of the given dtype, 2-D, row-major with unit
+    column stride and a row stride of at least its column count.  A wrong
+    shape or a transposed view would otherwise make the kernels read or
+    write outside the tensor."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: must be a CUDA tensor")
+    if dtype is not None and t.dtype != _torch_dtype(dtype):
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {_torch_dtype(dtype)}")
+    if t.dim() != 2:
+        raise ValueError(f"{name}: expected a 2-D matrix, got shape {tuple(t.shape)}")
+    if t.shape[1] > 1 and t.stride(1) != 1:
+        raise ValueError(f"{name}: columns must be contiguous (stride(1) == 1); pass .contiguous()")
+    if t.shape[0] > 1 and t.stride(0) < t.shape[1]:
+        raise ValueError(f"{name}: row stride {t.stride(0)} below the column count {t.shape[1]}")
+    if rows is not None and t.shape[0] != rows:
+        raise ValueError(f"{name}: {t.shape[0]} rows, expected {rows}")
+    if cols is not None and t.shape[1] != cols:
+        raise ValueError(f"{name}: {t.shape[1]} columns, expected {cols}")
+    return t
+
+
+def _ld(t):
+    return max(t.stride(0), t.shape[1], 1) if t.shape[0] > 1 else max(t.shape[1], 1)
+
+
+def _dev_vec(t, name, dtype, length):
+    """Checks a device vector argument: a contiguous CUDA tensor of the
+    given dtype with at least `length` elements."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: must be a CUDA tensor")
+    if t.dtype != _torch_dtype(dtype):
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {_torch_dtype(dtype)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if t.numel() < length:
+        raise ValueError(f"{name}: {t.numel()} elements, need {length}")
+    return t
+
+
+def _square(A, name, dtype):
+    _dev_mat(A, name, dtype)
+    if A.shape[0] != A.shape[1]:
+        raise ValueError(f"{name}: must be square, got {tuple(A.shape)}")
+    return A.shape[0]
+
+
+def _gemm_shapes(A, B, C, transa, transb, dtype):
+    """Checks op(A) op(B) -> C; dtype None leaves A and B's dtype to the caller
+    (C is then float32)."""
+    for t, nm in ((A, "A"), (B, "B")):
+        _dev_mat(t, nm, dtype)
+    _dev_mat(C, "C", dtype or "f32")
+    if transa not in ("N", "T") or transb not in ("N", "T"):
+        raise ValueError("trans flags must be 'N' or 'T'")
     m, n = C.shape
-    k = A.shape[0] if transa == "T" else A.shape[1]
+    am, ak = (A.shape[1], A.shape[0]) if transa == "T" else tuple(A.shape)
+    bk, bn = (B.shape[1], B.shape[0]) if transb == "T" else tuple(B.shape)
+    if am != m or bn != n or ak != bk:
+        raise ValueError(f"gemm: op(A) {am}x{ak}, op(B) {bk}x{bn}, C {m}x{n} do not conform")
+    return m, n, ak
+
+
+def dgemm_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", stream=None):
+    m, n, k = _gemm_shapes(A, B, C, transa, transb, "f64")
     _check(_lib.lmx_dgemm_dev(_stream(stream), transa.encode(), transb.encode(), m, n, k, alpha, _tp(A),
-                              A.stride(0), _tp(B), B.stride(0), beta, _tp(C), C.stride(0)))
+                              _ld(A), _tp(B), _ld(B), beta, _tp(C), _ld(C)))
 
 
 def sgemm_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", stream=None):
-    m, n = C.shape
-    k = A.shape[0] if transa == "T" else A.shape[1]
+    m, n, k = _gemm_shapes(A, B, C, transa, transb, "f32")
     _check(_lib.lmx_sgemm_dev(_stream(stream), transa.encode(), transb.encode(), m, n, k, alpha, _tp(A),
-                              A.stride(0), _tp(B), B.stride(0), beta, _tp(C), C.stride(0)))
+                              _ld(A), _tp(B), _ld(B), beta, _tp(C), _ld(C)))
+
+
+def gemm_bf16_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", stream=None):
+    """C (float32) = alpha op(A) op(B) + beta C, A and B torch.bfloat16."""
+    import torch
+    for t, nm in ((A, "A"), (B, "B")):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.bfloat16:
+            raise ValueError(f"{nm}: expected a torch.bfloat16 tensor")
+    m, n, k = _gemm_shapes(A, B, C, transa, transb, None)
+    _check(_lib.lmx_gemm_bf16_dev(_stream(stream), transa.encode(), transb.encode(), m, n, k, alpha, _tp(A),
+                                  _ld(A), _tp(B), _ld(B), beta, _tp(C), _ld(C)))
+
+
+def sgemm_bf16x3_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", stream=None):
+    """C = alpha op(A) op(B) + beta C for float32 tensors on BF16 matrix cores,
+    each operand split in two BF16 parts (hi*hi + hi*lo + lo*hi)."""
+    m, n, k = _gemm_shapes(A, B, C, transa, transb, "f32")
+    _check(_lib.lmx_sgemm_bf16x3_dev(_stream(stream), transa.encode(), transb.encode(), m, n, k, alpha, _tp(A),
+                                     _ld(A), _tp(B), _ld(B), beta, _tp(C), _ld(C)))
 
 
 def getrf_dev(A, ipiv, info, stream=None):
-    _check(_lib.lmx_dgetrf_dev(_stream(stream), A.shape[0], _tp(A), A.stride(0), _tp(ipiv), _tp(info)))
+    n = _square(A, "A", "f64")
+    _dev_vec(ipiv, "ipiv", "i32", n)
+    _dev_vec(info, "info", "i32", 1)
+    _check(_lib.lmx_dgetrf_dev(_stream(stream), n, _tp(A), _ld(A), _tp(ipiv), _tp(info)))
 
 
 def sgetrf_dev(A, ipiv, info, stream=None):
-    _check(_lib.lmx_sgetrf_dev(_stream(stream), A.shape[0], _tp(A), A.stride(0), _tp(ipiv), _tp(info)))
+    n = _square(A, "A", "f32")
+    _dev_vec(ipiv, "ipiv", "i32", n)
+    _dev_vec(info, "info", "i32", 1)
+    _check(_lib.lmx_sgetrf_dev(_stream(stream), n, _tp(A), _ld(A), _tp(ipiv), _tp(info)))
+
+
+def lu_dev(A, perm, info, stream=None):
+    """The generic lu on a device matrix: A <- packed factors, perm <- lu's
+    permutation vector (int32), with lu's tie rule."""
+    n = _square(A, "A", "f64")
+    _dev_vec(perm, "perm", "i32", n)
+    _dev_vec(info, "info", "i32", 1)
+    _check(_lib.lmx_dlu_dev(_stream(stream), n, _tp(A), _ld(A), _tp(perm), _tp(info)))
 
 
 def getrs_dev(LU, ipiv, B, stream=None):
-    _check(_lib.lmx_dgetrs_dev(_stream(stream), LU.shape[0], B.shape[1], _tp(LU), LU.stride(0), _tp(ipiv),
-                               _tp(B), B.stride(0)))
+    n = _square(LU, "LU", "f64")
+    _dev_mat(B, "B", "f64", rows=n)
+    _dev_vec(ipiv, "ipiv", "i32", n)
+    _check(_lib.lmx_dgetrs_dev(_stream(stream), n, B.shape[1], _tp(LU), _ld(LU), _tp(ipiv), _tp(B), _ld(B)))
 
 
 def gesv_dev(A, ipiv, B, info, stream=None):
-    _check(_lib.lmx_dgesv_dev(_stream(stream), A.shape[0], B.shape[1], _tp(A), A.stride(0), _tp(ipiv),
-                              _tp(B), B.stride(0), _tp(info)))
+    n = _square(A, "A", "f64")
+    _dev_mat(B, "B", "f64", rows=n)
+    _dev_vec(ipiv, "ipiv", "i32", n)
+    _dev_vec(info, "info", "i32", 1)
+    _check(_lib.lmx_dgesv_dev(_stream(stream), n, B.shape[1], _tp(A), _ld(A), _tp(ipiv), _tp(B), _ld(B),
+                              _tp(info)))
 
 
 def potrf_dev(A, info, stream=None):
-    _check(_lib.lmx_dpotrf_dev(_stream(stream), A.shape[0], _tp(A), A.stride(0), _tp(info)))
+    n = _square(A, "A", "f64")
+    _dev_vec(info, "info", "i32", 1)
+    _check(_lib.lmx_dpotrf_dev(_stream(stream), n, _tp(A), _ld(A), _tp(info)))
 
 
 def potrs_dev(G, B, stream=None):
-    _check(_lib.lmx_dpotrs_dev(_stream(stream), G.shape[0], B.shape[1], _tp(G), G.stride(0), _tp(B),
-                               B.stride(0)))
+    n = _square(G, "G", "f64")
+    _dev_mat(B, "B", "f64", rows=n)
+    _check(_lib.lmx_dpotrs_dev(_stream(stream), n, B.shape[1], _tp(G), _ld(G), _tp(B), _ld(B)))
 
 
 def posv_dev(A, B, info, stream=None):
-    _check(_lib.lmx_dposv_dev(_stream(stream), A.shape[0], B.shape[1], _tp(A), A.stride(0), _tp(B),
-                              B.stride(0), _tp(info)))
+    n = _square(A, "A", "f64")
+    _dev_mat(B, "B", "f64", rows=n)
+    _dev_vec(info, "info", "i32", 1)
+    _check(_lib.lmx_dposv_dev(_stream(stream), n, B.shape[1], _tp(A), _ld(A), _tp(B), _ld(B), _tp(info)))
 
 
 def trsm_dev(T, B, uplo="L", trans="N", diag="N", stream=None):
-    _check(_lib.lmx_dtrsm_dev(_stream(stream), uplo.encode(), trans.encode(), diag.encode(), T.shape[0],
-                              B.shape[1], _tp(T), T.stride(0), _tp(B), B.stride(0)))
+    n = _square(T, "T", "f64")
+    _dev_mat(B, "B", "f64", rows=n)
+    if uplo not in ("L", "U") or trans not in ("N", "T") or diag not in ("N", "U"):
+        raise ValueError("trsm flags: uplo L/U, trans N/T, diag N/U")
+    _check(_lib.lmx_dtrsm_dev(_stream(stream), uplo.encode(), trans.encode(), diag.encode(), n, B.shape[1],
+                              _tp(T), _ld(T), _tp(B), _ld(B)))
+
+
+def gemv_dev(A, x, y, alpha=1.0, beta=0.0, stream=None):
+    _dev_mat(A, "A", "f64")
+    m, n = A.shape
+    _dev_vec(x, "x", "f64", n)
+    _dev_vec(y, "y", "f64", m)
+    _check(_lib.lmx_dgemv_dev(_stream(stream), m, n, alpha, _tp(A), _ld(A), _tp(x), beta, _tp(y)))
+
+
+def norm_dev(A, which, out, stream=None):
+    _dev_mat(A, "A", "f64")
+    _dev_vec(out, "out", "f64", 1)
+    _check(_lib.lmx_dnorm_dev(_stream(stream), which.encode(), A.shape[0], A.shape[1], _tp(A), _ld(A), _tp(out)))
+
+
+def residual_dev(A, X, B, R, stream=None):
+    n = _square(A, "A", "f64")
+    _dev_mat(X, "X", "f64", rows=n)
+    k = X.shape[1]
+    _dev_mat(B, "B", "f64", rows=n, cols=k)
+    _dev_mat(R, "R", "f64", rows=n, cols=k)
+    _check(_lib.lmx_dresidual_dev(_stream(stream), n, k, _tp(A), _ld(A), _tp(X), _ld(X), _tp(B), _ld(B), _tp(R),
+                                  _ld(R)))
+
+
+def scaled_residual_dev(A, X, B, eta, stream=None):
+    n = _square(A, "A", "f64")
+    _dev_mat(X, "X", "f64", rows=n)
+    k = X.shape[1]
+    _dev_mat(B, "B", "f64", rows=n, cols=k)
+    _dev_vec(eta, "eta", "f64", k)
+    _check(_lib.lmx_dscaled_residual_dev(_stream(stream), n, k, _tp(A), _ld(A), _tp(X), _ld(X), _tp(B), _ld(B),
+                                         _tp(eta)))
+
+
+def syrk_dev(A, C, trans="N", alpha=1.0, beta=0.0, stream=None):
+    _dev_mat(A, "A", "f64")
+    n, k = (A.shape[1], A.shape[0]) if trans == "T" else tuple(A.shape)
+    _dev_mat(C, "C", "f64", rows=n, cols=n)
+    _check(_lib.lmx_dsyrk_dev(_stream(stream), trans.encode(), n, k, alpha, _tp(A), _ld(A), beta, _tp(C), _ld(C)))
+
+
+def det_dev(A, out, stream=None):
+    n = _square(A, "A", "f64")
+    _dev_vec(out, "out", "f64", 1)
+    _check(_lib.lmx_ddet_dev(_stream(stream), n, _tp(A), _ld(A), _tp(out)))
+
+
+def _mixed_args(A, B, X):
+    n = _square(A, "A", "f64")
+    _dev_mat(B, "B", "f64", rows=n)
+    _dev_mat(X, "X", "f64", rows=n, cols=B.shape[1])
+    return n, B.shape[1]
 
 
 def gesv_mixed_dev(A, B, X, precision=PREC_F32, stream=None):
+    n, k = _mixed_args(A, B, X)
     ri = RefineInfo()
-    _check(_lib.lmx_dgesv_mixed_dev(_stream(stream), precision, A.shape[0], B.shape[1], _tp(A), A.stride(0),
-                                    _tp(B), B.stride(0), _tp(X), X.stride(0), ctypes.byref(ri)), allow_info=True)
+    _check(_lib.lmx_dgesv_mixed_dev(_stream(stream), precision, n, k, _tp(A), _ld(A), _tp(B), _ld(B), _tp(X),
+                                    _ld(X), ctypes.byref(ri)), allow_info=True)
     return ri
 
 
 def posv_mixed_dev(A, B, X, precision=PREC_F32, stream=None):
+    n, k = _mixed_args(A, B, X)
     ri = RefineInfo()
-    _check(_lib.lmx_dposv_mixed_dev(_stream(stream), precision, A.shape[0], B.shape[1], _tp(A), A.stride(0),
-                                    _tp(B), B.stride(0), _tp(X), X.stride(0), ctypes.byref(ri)), allow_info=True)
+    _check(_lib.lmx_dposv_mixed_dev(_stream(stream), precision, n, k, _tp(A), _ld(A), _tp(B), _ld(B), _tp(X),
+                                    _ld(X), ctypes.byref(ri)), allow_info=True)
     return ri
 
 

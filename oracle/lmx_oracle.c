@@ -745,3 +745,250 @@ ORC_API void orc_chol_solve_factored(ix n, const double *G, double *x)
     orc_forward_chol_packed(n, G, x);
     orc_back_cholT_packed(n, G, x);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Level 1/2, norms, det, inverse iteration                                 */
+/* ------------------------------------------------------------------------ */
+
+/* rawDot# (Internal/Kernel.hs:89-110): four FMA lanes over i = 0, 4, 8, ...,
+ * reduced ((a + b) + c) + d, then the tail i >= 4 floor(n/4) with
+ * acc + x * y (a separate multiply and add). */
+ORC_API double orc_raw_dot(ix n, const double *x, const double *y)
+{
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    ix n4 = n - n % 4, i;
+    for (i = 0; i < n4; i += 4)
+        for (int l = 0; l < 4; ++l)
+            acc[l] = fma(x[i + l], y[i + l], acc[l]);
+    double s = ((acc[0] + acc[1]) + acc[2]) + acc[3];
+    for (i = n4; i < n; ++i) {
+        double p = x[i] * y[i];
+        s = s + p;
+    }
+    return s;
+}
+
+/* matvecP (BLAS/Level2.hs:132-140) through rawGemv (Internal/Kernel.hs:
+ * 119-134): y_i = rawDot#(row i of A, x). */
+ORC_API void orc_matvecp(ix m, ix n, const double *A, const double *x, double *y)
+{
+    for (ix i = 0; i < m; ++i)
+        y[i] = orc_raw_dot(n, A + i * n, x);
+}
+
+/* The generic gemv (BLAS/Level2.hs:92-100): a_i . x as foldl (acc + a x)
+ * from 0 in ascending j, then alpha * axi + beta * y_i. */
+ORC_API void orc_gemv_generic(ix m, ix n, double alpha, const double *A, const double *x, double beta,
+                              const double *y, double *out)
+{
+    for (ix i = 0; i < m; ++i) {
+        double acc = 0.0;
+        for (ix j = 0; j < n; ++j) {
+            double p = A[i * n + j] * x[j];
+            acc = acc + p;
+        }
+        double a = alpha * acc, b = beta * y[i];
+        out[i] = a + b;
+    }
+}
+
+/* norm1 (Norms.hs:124-130): column sums foldl (acc + |a_ij|) over i, the
+ * maximum over columns; normInf (:142-148) the same over rows; normFrob
+ * (:110-112) sqrt of foldl (acc + x * x) over the array in row-major order.
+ * `maximum` of a list keeps its running maximum with max (which keeps the
+ * first argument unless it is <= the second); NaN inputs are not restated. */
+ORC_API double orc_norm1(ix m, ix n, const double *A)
+{
+    double best = 0.0;
+    for (ix j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (ix i = 0; i < m; ++i) s = s + fabs(A[i * n + j]);
+        if (j == 0 || best <= s) best = s;
+    }
+    return best;
+}
+
+ORC_API double orc_norm_inf(ix m, ix n, const double *A)
+{
+    double best = 0.0;
+    for (ix i = 0; i < m; ++i) {
+        double s = 0.0;
+        for (ix j = 0; j < n; ++j) s = s + fabs(A[i * n + j]);
+        if (i == 0 || best <= s) best = s;
+    }
+    return best;
+}
+
+ORC_API double orc_norm_frob(ix m, ix n, const double *A)
+{
+    double s = 0.0;
+    for (ix e = 0; e < m * n; ++e) {
+        double p = A[e] * A[e];
+        s = s + p;
+    }
+    return sqrt(s);
+}
+
+/* det (Solve/LU.hs:534-558): the generic lu, the product of U's diagonal as
+ * foldl' (acc * u_ii) from 1, and (-1)^countSwaps of lu's permutation;
+ * countSwaps moves each p into place by one swap, so its count is
+ * n - (number of cycles), whose parity is the permutation's. */
+ORC_API double orc_det(ix n, const double *A)
+{
+    double *LU = malloc(sizeof(double) * (size_t)(n > 0 ? n * n : 1));
+    ix *piv = malloc(sizeof(ix) * (size_t)(n > 0 ? n : 1));
+    memcpy(LU, A, sizeof(double) * (size_t)(n * n));
+    orc_lu_generic(n, LU, piv);
+    double prod = 1.0;
+    for (ix i = 0; i < n; ++i) prod = prod * LU[i * n + i];
+    /* countSwaps restated: walk the list, swapping p into place */
+    ix nswaps = 0;
+    for (ix i = 0; i < n; ++i) {
+        if (piv[i] == i) continue;
+        ix p = piv[i];
+        for (ix t = i + 1; t < n; ++t)
+            if (piv[t] == i) piv[t] = p;
+        ++nswaps;
+    }
+    free(LU);
+    free(piv);
+    return (nswaps % 2 == 0) ? prod : -prod;
+}
+
+/* nrm2 (BLAS/Level1.hs:179-181): sqrt of foldl (acc + x * x). */
+static double nrm2_seq(ix n, const double *x)
+{
+    double s = 0.0;
+    for (ix i = 0; i < n; ++i) {
+        double p = x[i] * x[i];
+        s = s + p;
+    }
+    return sqrt(s);
+}
+
+/* inverseIteration (Eigen/Power.hs:113-135) at P Double: aShifted = A - mu I;
+ * z = luSolve aShifted q (the generic lu, forwardSubUnit, backSub),
+ * qNew = scal (1 / nrm2 z) z, lambda = dot qNew (matvec A qNew) with dot and
+ * matvec at their P Double specialisations (rawDot#), stopping when
+ * |lambda - prevLambda| < tol, or after max_iter steps with (prevLambda, q).
+ * Returns the steps taken. */
+ORC_API ix orc_inverse_iteration(ix n, const double *A, double mu, const double *q0, ix max_iter, double tol,
+                                 double *lambda, double *q)
+{
+    double *S = malloc(sizeof(double) * (size_t)(n > 0 ? n * n : 1));
+    double *z = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *t = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (ix i = 0; i < n; ++i)
+        for (ix j = 0; j < n; ++j) S[i * n + j] = i == j ? A[i * n + j] - mu : A[i * n + j];
+    memcpy(q, q0, sizeof(double) * (size_t)n);
+    double prev = 0.0;
+    ix it = 0;
+    for (; it < max_iter; ++it) {
+        orc_lusolve_generic(n, S, q, z);
+        double zn = nrm2_seq(n, z), r = 1.0 / zn;
+        for (ix i = 0; i < n; ++i) z[i] = z[i] * r;
+        orc_matvecp(n, n, A, z, t);
+        double lam = orc_raw_dot(n, z, t);
+        memcpy(q, z, sizeof(double) * (size_t)n);
+        if (fabs(lam - prev) < tol) {
+            prev = lam;
+            ++it;
+            break;
+        }
+        prev = lam;
+    }
+    *lambda = prev;
+    free(S);
+    free(z);
+    free(t);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* matMulPPar: the all-cores GEMM                                           */
+/* ------------------------------------------------------------------------ */
+#include <pthread.h>
+
+/* gridDims (BLAS/Level3.hs:276-294): the factorisation pr x pc = p whose
+ * aspect pr/pc is nearest m/n, the first best among i = 1..floor(sqrt p),
+ * then its transpose if that scores strictly better. */
+static void grid_dims(ix p, ix m, ix n, ix *pr_out, ix *pc_out)
+{
+    ix sq = (ix)floor(sqrt((double)p));
+    double target = (double)m / (double)(n > 1 ? n : 1);
+    ix bpr = 1, bpc = p;
+    double bscore = -1.0;
+    for (ix i = 1; i <= sq; ++i) {
+        if (p % i) continue;
+        ix pr = i, pc = p / i;
+        double sc = fabs((double)pr / (double)(pc > 1 ? pc : 1) - target);
+        if (bscore < 0.0 || bscore > sc) {
+            bscore = sc;
+            bpr = pr;
+            bpc = pc;
+        }
+    }
+    double tscore = fabs((double)bpc / (double)(bpr > 1 ? bpr : 1) - target);
+    if (tscore < bscore) {
+        ix t = bpr;
+        bpr = bpc;
+        bpc = t;
+    }
+    *pr_out = bpr;
+    *pc_out = bpc;
+}
+
+struct par_slice {
+    ix m, k, n, i0, i1, j0, j1;
+    const double *A, *B;
+    double *C;
+};
+
+static void *par_worker(void *arg)
+{
+    struct par_slice *s = arg;
+    if (s->i1 > s->i0 && s->j1 > s->j0)
+        gemm_acc_rawkernel(s->i1 - s->i0, s->k, s->j1 - s->j0, s->A + s->i0 * s->k, s->k, s->B + s->j0, s->n,
+                           s->C + s->i0 * s->n + s->j0, s->n);
+    return NULL;
+}
+
+/* matMulPPar (BLAS/Level3.hs:213-272) with `caps` threads: at least 16 rows
+ * a thread; a pr x pc grid of (row, column) slices (gridDims) when there are
+ * >= 4 threads and m, n >= 128, else row slices; each slice is the
+ * reference's slice kernel, whose k tiles and column tails start at the
+ * slice's own first column. */
+ORC_API void orc_matmulp_par(ix m, ix k, ix n, const double *A, const double *B, double *C, ix caps)
+{
+    ix maxt = m / 16 > 1 ? m / 16 : 1;
+    ix nt = caps < m ? caps : m;
+    if (nt > maxt) nt = maxt;
+    if (nt <= 1) {
+        orc_matmulp(m, k, n, A, B, C);
+        return;
+    }
+    memset(C, 0, (size_t)(m * n) * sizeof(double));
+    ix pr = nt, pc = 1;
+    if (nt >= 4 && m >= 128 && n >= 128) grid_dims(nt, m, n, &pr, &pc);
+    ix rch = (m + pr - 1) / pr, cch = (n + pc - 1) / pc;
+    struct par_slice *sl = malloc(sizeof(struct par_slice) * (size_t)(pr * pc));
+    pthread_t *th = malloc(sizeof(pthread_t) * (size_t)(pr * pc));
+    for (ix tr = 0; tr < pr; ++tr)
+        for (ix tc = 0; tc < pc; ++tc) {
+            struct par_slice *s = &sl[tr * pc + tc];
+            s->m = m;
+            s->k = k;
+            s->n = n;
+            s->i0 = tr * rch;
+            s->i1 = min_ix(s->i0 + rch, m);
+            s->j0 = tc * cch;
+            s->j1 = min_ix(s->j0 + cch, n);
+            s->A = A;
+            s->B = B;
+            s->C = C;
+            pthread_create(&th[tr * pc + tc], NULL, par_worker, s);
+        }
+    for (ix t = 0; t < pr * pc; ++t) pthread_join(th[t], NULL);
+    free(sl);
+    free(th);
+}

@@ -72,11 +72,21 @@ def lib():
             "orc_back_cholT_packed": [_ix, _pd, _pd],
             "orc_cholsolvep": [_ix, _pd, _pd, _pd, _pd],
             "orc_chol_solve_factored": [_ix, _pd, _pd],
+            "orc_matvecp": [_ix, _ix, _pd, _pd, _pd],
+            "orc_gemv_generic": [_ix, _ix, ctypes.c_double, _pd, _pd, ctypes.c_double, _pd, _pd],
+            "orc_inverse_iteration": [_ix, _pd, ctypes.c_double, _pd, _ix, ctypes.c_double, _pd, _pd],
+            "orc_matmulp_par": [_ix, _ix, _ix, _pd, _pd, _pd, _ix],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = _ix if name.endswith("_factor") else None
+            fn.restype = _ix if name.endswith("_factor") or name == "orc_inverse_iteration" else None
+        for name, args in {"orc_raw_dot": [_ix, _pd, _pd], "orc_norm1": [_ix, _ix, _pd],
+                           "orc_norm_inf": [_ix, _ix, _pd], "orc_norm_frob": [_ix, _ix, _pd],
+                           "orc_det": [_ix, _pd]}.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -353,3 +363,86 @@ def scaled_residual_cholesky(A, G):
 def condition_tolerance(kappa, n):
     """conditionTolerance (test/Test/Residuals.hs:218-219): kappa n eps."""
     return kappa * n * MACHINE_EPS
+
+
+# --------------------------------------------------------------------------
+# Level 1/2, norms, SYRK, det, inverse iteration
+# --------------------------------------------------------------------------
+
+def rawDot(x, y):
+    """rawDot# (Internal/Kernel.hs:89-110)."""
+    x, px = _d(x)
+    y, py = _d(y)
+    return lib().orc_raw_dot(len(x), px, py)
+
+
+def matvecP(A, x):
+    """matvecP (BLAS/Level2.hs:132-140): y_i = rawDot#(row i, x)."""
+    A, pa = _d(A)
+    x, px = _d(x)
+    y = np.empty(A.shape[0])
+    lib().orc_matvecp(A.shape[0], A.shape[1], pa, px, _p(y))
+    return y
+
+
+def gemv(alpha, A, x, beta, y):
+    """The generic gemv (BLAS/Level2.hs:92-100)."""
+    A, pa = _d(A)
+    x, px = _d(x)
+    y, py = _d(y)
+    out = np.empty(A.shape[0])
+    lib().orc_gemv_generic(A.shape[0], A.shape[1], alpha, pa, px, beta, py, _p(out))
+    return out
+
+
+def norm1(A):
+    A, pa = _d(np.atleast_2d(A))
+    return lib().orc_norm1(A.shape[0], A.shape[1], pa)
+
+
+def normInf(A):
+    A, pa = _d(np.atleast_2d(A))
+    return lib().orc_norm_inf(A.shape[0], A.shape[1], pa)
+
+
+def normFrob(A):
+    A, pa = _d(np.atleast_2d(A))
+    return lib().orc_norm_frob(A.shape[0], A.shape[1], pa)
+
+
+def matMulAtAP(A):
+    """matMulAtAP (BLAS/Level3.hs:353-373): rawSyrkLowerKernel's lower
+    triangle has, element by element, the rounding of rawGemmKernel on
+    (A^T, A) (the same 64-tiles over k and columns, FMA chains, and the
+    scalar column tail per k tile), mirrored into the upper triangle."""
+    A = np.asarray(A, dtype=np.float64)
+    C = matMulP(np.ascontiguousarray(A.T), A)
+    L = np.tril(C)
+    return L + np.tril(C, -1).T
+
+
+def det(A):
+    """det (Solve/LU.hs:534-558)."""
+    A, pa = _d(A)
+    return lib().orc_det(A.shape[0], pa)
+
+
+def inverseIteration(A, mu, q0, max_iter, tol):
+    """inverseIteration (Eigen/Power.hs:113-135) -> (lambda, q, steps)."""
+    A, pa = _d(A)
+    q0, pq = _d(q0)
+    q = np.empty(A.shape[0])
+    lam = ctypes.c_double(0.0)
+    it = lib().orc_inverse_iteration(A.shape[0], pa, mu, pq, max_iter, tol, ctypes.pointer(lam), _p(q))
+    return lam.value, q, it
+
+
+def matMulPPar(A, B, threads):
+    """matMulPPar (BLAS/Level3.hs:213-294) on `threads` host threads."""
+    A, pa = _d(A)
+    B, pb = _d(B)
+    m, k = A.shape
+    n = B.shape[1]
+    C = np.empty((m, n))
+    lib().orc_matmulp_par(m, k, n, pa, pb, _p(C), threads)
+    return C

@@ -66,3 +66,36 @@ def swaps_to_perm(ipiv):
 @pytest.fixture(scope="session")
 def known_pivots():
     return known_pivot_matrix
+
+
+def tie_block_matrix(n, seed, zero_cols=0):
+    """-> A (n x n) whose nonzeros form 2 x 2 blocks [[a, b], [c, d]] with
+    |a| = |c| and |b| = |d| (so each pivot search in a block column is a tie
+    between the block's two rows) and 1 x 1 blocks, scattered over the matrix
+    by independent row and column permutations, plus `zero_cols` empty
+    row/column pairs (a zero column is an n-way tie at 0, and the matrix is
+    singular).  The signs make every 2 x 2 block nonsingular.  Every
+    multiplier of partial pivoting is 0 or +-1 and every product and partial
+    sum is a small integer, so any order of the same operations gives the
+    same bits: the pivot choices alone decide the factors."""
+    rng = np.random.default_rng(seed)
+    vals = np.array([1.0, 2.0, 3.0, 4.0])
+    m = n - zero_cols
+    nb2 = m // 3  # 2 x 2 blocks; the rest are 1 x 1
+    B = np.zeros((n, n))
+    i = 0
+    for _ in range(nb2):
+        s, t = rng.choice(vals, 2)
+        g1, g2, g3 = rng.choice([-1.0, 1.0], 3)
+        B[i:i + 2, i:i + 2] = [[g1 * s, g3 * t], [g2 * s, -g1 * g2 * g3 * t]]
+        i += 2
+    while i < m:
+        B[i, i] = rng.choice([-1.0, 1.0]) * rng.choice(vals)
+        i += 1
+    rows, cols = rng.permutation(n), rng.permutation(n)
+    return np.ascontiguousarray(B[rows][:, cols])
+
+
+@pytest.fixture(scope="session")
+def tie_blocks():
+    return tie_block_matrix

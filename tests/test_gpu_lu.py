@@ -186,3 +186,146 @@ def test_device_gesv_large(lmx_gpu, orc, n):
     # pivots are a valid swap list
     iv = ipiv.cpu().numpy()
     assert np.all(iv >= np.arange(n)) and np.all(iv < n)
+
+
+# ---- the generic lu: maximumBy's tie rule (Solve/LU.hs:121-167) ----------
+
+def test_lu_tie_kat_2x2(lmx_gpu, orc):
+    # maximumBy keeps the later of equal |a_ik| (Solve/LU.hs:128-135): the
+    # reference's lu gives perm [1, 0] and LU [[-1, 3], [-1, 5]] here
+    A = np.array([[1.0, 2.0], [-1.0, 3.0]])
+    LU, perm = lmx_gpu.lu(A)
+    np.testing.assert_array_equal(perm, [1, 0])
+    np.testing.assert_array_equal(LU, [[-1.0, 3.0], [-1.0, 5.0]])
+    LUo, permo = orc.lu(A)
+    np.testing.assert_array_equal(perm, permo)
+    np.testing.assert_array_equal(LU, LUo)
+    LUf, permf = lmx_gpu.lu_float(A)
+    np.testing.assert_array_equal(permf, [1, 0])
+    np.testing.assert_array_equal(LUf, LUo.astype(np.float32))
+
+
+@pytest.mark.parametrize("n,zero_cols", [(7, 0), (33, 1), (64, 0), (300, 2), (1000, 0), (2100, 3)])
+def test_lu_ties_bit_exact_vs_oracle(lmx_gpu, orc, tie_blocks, n, zero_cols):
+    # every block column is a tie between two rows scattered over the panel
+    # workgroups (512 rows each); zero columns are n-way ties at 0 and skip
+    # their elimination.  Exact arithmetic: the factors must be the oracle's bits.
+    A = tie_blocks(n, 7 * n + zero_cols, zero_cols)
+    LU, perm = lmx_gpu.lu(A)
+    LUo, permo = orc.lu(A)
+    np.testing.assert_array_equal(perm, permo)
+    np.testing.assert_array_equal(LU, LUo)
+    # the same matrix under luSolveP's rule (getrf): the first row of a tie.
+    # (With a zero column luSolveP's packed path divides by zero and its
+    # later pivots see NaN, where getrf reports info and skips the column,
+    # so that comparison is made on the nonsingular matrices only.)
+    LUg, ipiv, info = lmx_gpu.getrf(A)
+    if zero_cols == 0:
+        LUp, swp = orc.lu_p_factor(A)
+        assert info == 0
+        np.testing.assert_array_equal(ipiv, swp)
+        np.testing.assert_array_equal(LUg, LUp)
+        if n >= 33:
+            assert not np.array_equal(lmx_gpu.swaps_to_perm(ipiv), perm)
+    else:
+        assert info > 0
+
+
+@pytest.mark.parametrize("n", [64, 1000])
+def test_lu_float_ties_bit_exact(lmx_gpu, orc, tie_blocks, n):
+    # lu at e = Float: the integer data stays exact in FP32
+    A = tie_blocks(n, 3 * n)
+    LU, perm = lmx_gpu.lu_float(A)
+    LUo, permo = orc.lu(A)
+    np.testing.assert_array_equal(perm, permo)
+    np.testing.assert_array_equal(LU, LUo.astype(np.float32))
+
+
+def test_lu_nan_rule_vs_oracle(lmx_gpu, orc, tie_blocks):
+    # maximumBy never moves past a NaN candidate to a later row and never
+    # leaves a NaN at row k (compare gives GT whenever NaN is involved)
+    A = tie_blocks(40, 11)
+    c = int(np.flatnonzero(A[:, 0])[0])  # a row of column 0's tie
+    below = [i for i in range(40) if A[i, 0] == 0 and i > 0]
+    B = A.copy()
+    B[below[-1], 0] = np.nan            # NaN below row 0: never chosen
+    C = A.copy()
+    C[0, 0] = np.nan                    # NaN on the diagonal: kept
+    for M in (B, C):
+        LU, perm = lmx_gpu.lu(M)
+        LUo, permo = orc.lu(M)
+        np.testing.assert_array_equal(perm, permo)
+        np.testing.assert_array_equal(LU, LUo)
+    assert c >= 0
+
+
+@pytest.mark.parametrize("n", [3, 10, 33, 64, 100, 130])
+def test_lu_golden_vs_oracle(lmx_gpu, orc, golden, n):
+    # random data has no ties: lu's pivots are LAPACK's and the oracle's
+    A = golden[f"A_{n}"]
+    LU, perm = lmx_gpu.lu(A)
+    LUo, permo = orc.lu(A)
+    np.testing.assert_array_equal(perm, permo)
+    np.testing.assert_allclose(LU, LUo, rtol=0, atol=1e-12 * n)
+
+
+def test_lu_dev_matches_host(lmx_gpu, orc, tie_blocks):
+    import torch
+    n = 700
+    A = tie_blocks(n, 5, 1)
+    dA = torch.tensor(A, device="cuda")
+    perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lmx_gpu.lu_dev(dA, perm, info)
+    torch.cuda.synchronize()
+    LUo, permo = orc.lu(A)
+    np.testing.assert_array_equal(perm.cpu().numpy(), permo)
+    np.testing.assert_array_equal(dA.cpu().numpy(), LUo)
+    assert info.item() > 0  # the zero column is reported (1-based)
+
+
+@pytest.mark.parametrize("n", [16384, 20000])
+def test_getrs_dev_large_swap_list(lmx_gpu, n):
+    # the swap-list conversion of getrs_dev above n = 16384
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(n)
+    A = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    B = torch.rand(n, 2, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    LU = A.clone()
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lmx_gpu.getrf_dev(LU, ipiv, info)
+    X = B.clone()
+    lmx_gpu.getrs_dev(LU, ipiv, X)
+    torch.cuda.synchronize()
+    assert info.item() == 0
+    R = B - A @ X
+    num = R.abs().max(dim=0).values
+    den = A.abs().sum(1).max() * X.abs().max(dim=0).values + B.abs().max(dim=0).values
+    assert (num / den).max().item() <= 10 * n * EPS
+
+
+def test_dev_wrappers_refuse_bad_tensors(lmx_gpu):
+    # checked in Python before the pointers reach the library
+    import torch
+    n = 64
+    A = torch.rand(n, n, dtype=torch.float64, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    with pytest.raises(ValueError, match="dtype"):
+        lmx_gpu.getrf_dev(A.float(), ipiv, info)
+    with pytest.raises(ValueError, match="square"):
+        lmx_gpu.getrf_dev(A[:, :32], ipiv, info)
+    with pytest.raises(ValueError, match="stride"):
+        lmx_gpu.getrf_dev(A.t(), ipiv, info)
+    with pytest.raises(ValueError, match="elements"):
+        lmx_gpu.getrf_dev(A, ipiv[:10], info)
+    with pytest.raises(ValueError, match="dtype"):
+        lmx_gpu.getrf_dev(A, ipiv.long(), info)
+    with pytest.raises(ValueError, match="rows"):
+        lmx_gpu.getrs_dev(A, ipiv, torch.zeros(n + 1, 2, dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError, match="conform"):
+        lmx_gpu.dgemm_dev(A, A[:32], torch.zeros(n, n, dtype=torch.float64, device="cuda"))
+    X = torch.zeros(n, 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="columns"):
+        lmx_gpu.gesv_mixed_dev(A, torch.zeros(n, 2, dtype=torch.float64, device="cuda"), X)

@@ -98,3 +98,53 @@ def test_device_mixed_large(lmx_gpu):
     num = R.abs().max(dim=0).values
     den = A.abs().sum(1).max() * X.abs().max(dim=0).values + B.abs().max(dim=0).values
     assert (num / den).max().item() <= 10 * n * EPS
+
+
+# ---- the BF16 factor (trailing updates on BF16 tiles, GMRES-IR) ----------
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+@pytest.mark.parametrize("n,nrhs", [(300, 1), (1000, 4)])
+def test_gesv_mixed_bf16(lmx_gpu, orc, kappa, n, nrhs):
+    A = randsvd(n, kappa, int(kappa) + n)
+    B = np.random.default_rng(n).uniform(-1, 1, (n, nrhs))
+    X, ri = lmx_gpu.gesv_mixed(A, B, lmx_gpu.PREC_BF16)
+    assert ri.converged == 1 and ri.fell_back == 0
+    assert orc.scaled_residual_linear(A, X, B) <= 10 * n * EPS
+    X64 = sl.solve(A, B)
+    err = np.abs(X - X64).max() / np.abs(X64).max()
+    assert err <= 10 * orc.condition_tolerance(kappa, n)
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+@pytest.mark.parametrize("n,nrhs", [(300, 1), (1000, 4)])
+def test_posv_mixed_bf16(lmx_gpu, orc, kappa, n, nrhs):
+    A = randsvd(n, kappa, int(kappa) + n + 1, spd=True)
+    A = (A + A.T) / 2
+    B = np.random.default_rng(n + 1).uniform(-1, 1, (n, nrhs))
+    X, ri = lmx_gpu.posv_mixed(A, B, lmx_gpu.PREC_BF16)
+    assert ri.converged == 1 and ri.fell_back == 0
+    assert orc.scaled_residual_linear(A, X, B) <= 10 * n * EPS
+    X64 = sl.solve(A, B, assume_a="pos")
+    err = np.abs(X - X64).max() / np.abs(X64).max()
+    assert err <= 10 * orc.condition_tolerance(kappa, n)
+
+
+def test_bf16_falls_back_beyond_range(lmx_gpu, orc):
+    n = 200
+    A = randsvd(n, 1e13, 4)
+    b = np.random.default_rng(4).uniform(-1, 1, n)
+    x, ri = lmx_gpu.gesv_mixed(A, b, lmx_gpu.PREC_BF16)
+    assert ri.fell_back == 1
+    assert orc.scaled_residual_linear(A, x, b) <= 10 * n * EPS
+
+
+def test_bf16_factor_is_not_the_fp32_factor(lmx_gpu):
+    # the BF16 trailing updates change the factor (and the refinement path):
+    # the iteration counts differ from the FP32 factor's on the same matrix
+    n = 1000
+    A = randsvd(n, 1e4, 11)
+    b = np.random.default_rng(11).uniform(-1, 1, (n, 1))
+    _, r32 = lmx_gpu.gesv_mixed(A, b, lmx_gpu.PREC_F32)
+    _, r16 = lmx_gpu.gesv_mixed(A, b, lmx_gpu.PREC_BF16)
+    assert r32.converged == 1 and r16.converged == 1
+    assert (r32.iterations, r32.backward_error) != (r16.iterations, r16.backward_error)

@@ -167,3 +167,61 @@ def test_server_concurrent_clients(lmx_gpu, orc):
     assert not errors
     for (k, A, B), X in zip(reqs, Xs):
         check(orc, k, A, B, X)
+
+
+def test_server_concurrent_large_lu(lmx_gpu, orc):
+    # several n >= 4096 LU factorisations at once (8-workgroup panels on
+    # concurrent streams, beside a Cholesky request): all complete, and each
+    # gives the same bits as alone
+    rng = np.random.default_rng(44)
+    reqs = [(LU, rng.uniform(-1, 1, (n, n)), rng.uniform(-1, 1, (n, 2))) for n in (4096, 4100, 4500, 5000)]
+    reqs.append((CH, spd(4096, rng), rng.uniform(-1, 1, (4096, 1))))
+    srv = lmx_gpu.Server(slots=5)
+    try:
+        Xs = [np.zeros_like(B) for _, _, B in reqs]
+        ids = [srv.submit(k, B, X, A=A) for (k, A, B), X in zip(reqs, Xs)]
+        res = [srv.wait(i) for i in ids]
+    finally:
+        srv.close()
+    assert all(r.rc == 0 for r in res), [r.rc for r in res]
+    for (k, A, B), X in zip(reqs, Xs):
+        assert orc.scaled_residual_linear(A, X, B) <= 10 * A.shape[0] * EPS
+    alone = lmx_gpu.Server(slots=1)
+    try:
+        k, A, B = reqs[0]
+        Y = np.zeros_like(B)
+        assert alone.wait(alone.submit(k, B, Y, A=A)).rc == 0
+        np.testing.assert_array_equal(Y, Xs[0])
+    finally:
+        alone.close()
+
+
+def test_server_second_waiter_is_refused(lmx_gpu):
+    # only one thread may wait for a request; a second one gets an error
+    # instead of touching the request after the first has collected it
+    rng = np.random.default_rng(5)
+    n = 2000
+    A, B = rng.uniform(-1, 1, (n, n)), rng.uniform(-1, 1, (n, 1))
+    X = np.zeros_like(B)
+    srv = lmx_gpu.Server(slots=1)
+    try:
+        rid = srv.submit(LU, B, X, A=A)
+        out = {}
+
+        def waiter(tag):
+            try:
+                out[tag] = srv.wait(rid).rc
+            except lmx_gpu.LmxError as e:
+                out[tag] = e.code
+
+        t1 = threading.Thread(target=waiter, args=("a",))
+        t1.start()
+        import time
+        time.sleep(0.05)
+        t2 = threading.Thread(target=waiter, args=("b",))
+        t2.start()
+        t1.join()
+        t2.join()
+    finally:
+        srv.close()
+    assert sorted(out.values()) == [lmx_gpu.LMX_ERR_ARG, 0]

@@ -140,6 +140,45 @@ def test_lu_tie_breaking(orc):
     np.testing.assert_array_equal(LUp, [[1, 2], [-1, 5]])
 
 
+def _lu_last_max_replay(A):
+    """Outer-product LU with partial pivoting (GVL4 Algorithm 3.4.1) written
+    with numpy row operations; the pivot is the last row of largest |a_ik|
+    (argmax over the reversed column).  On tie-block matrices every value is
+    an integer, which is asserted, so the replay is exact."""
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    perm = np.arange(n)
+    for k in range(n - 1):
+        col = np.abs(A[k:, k])
+        p = k + (len(col) - 1 - int(np.argmax(col[::-1])))
+        A[[k, p]] = A[[p, k]]
+        perm[[k, p]] = perm[[p, k]]
+        if A[k, k] != 0:
+            mult = A[k + 1:, k] / A[k, k]
+            A[k + 1:, k] = mult
+            A[k + 1:, k + 1:] -= np.outer(mult, A[k, k + 1:])
+    return A, perm
+
+
+@pytest.mark.parametrize("n,zero_cols", [(2, 0), (7, 0), (12, 1), (40, 0), (90, 3)])
+def test_lu_tie_blocks_exact(orc, tie_blocks, n, zero_cols):
+    # every pivot search of a block column is a tie; the generic lu takes the
+    # later row (maximumBy), and the arithmetic is exact integer work
+    A = tie_blocks(n, 100 + n, zero_cols)
+    LU, perm = orc.lu(A)
+    LUr, permr = _lu_last_max_replay(A)
+    np.testing.assert_array_equal(perm, permr)
+    np.testing.assert_array_equal(LU, LUr)
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    assert np.all(np.isin(np.tril(LU, -1), [-1.0, 0.0, 1.0]))
+    np.testing.assert_array_equal(A[perm], L @ U)
+    # the first-max rule of luSolveP picks other rows on the same matrix
+    if n >= 7:
+        _, sw = orc.lu_p_factor(A)
+        assert not np.array_equal(orc.swaps_to_perm(sw), perm)
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 10, 12, 33, 63, 64, 65, 100, 130])
 def test_lu_p_factor_vs_lapack(orc, golden, n):
     A = golden[f"A_{n}"]
@@ -330,3 +369,129 @@ def test_cholesky_reads_lower_triangle_only(orc):
     b = rng.uniform(-1, 1, n)
     np.testing.assert_array_equal(orc.choleskySolveP(junk, b), orc.choleskySolveP(S, b))
     np.testing.assert_array_equal(orc.cholesky(junk), orc.cholesky(S))
+
+
+# ---- level 1/2, norms, SYRK, det, inverse iteration ----------------------
+
+from fractions import Fraction  # noqa: E402
+
+
+def _fma_exact(a, b, c):
+    # one rounding of the exact a*b + c: the semantics of fma()
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def _raw_dot_emulated(x, y):
+    # rawDot# (Internal/Kernel.hs:89-110) in exact arithmetic with explicit
+    # roundings: four FMA lanes, ((a+b)+c)+d, then mul+add for the tail
+    n = len(x)
+    n4 = n - n % 4
+    acc = [0.0] * 4
+    for i in range(0, n4, 4):
+        for lane in range(4):
+            acc[lane] = _fma_exact(x[i + lane], y[i + lane], acc[lane])
+    s = float(Fraction(float(Fraction(acc[0]) + Fraction(acc[1]))) + Fraction(acc[2]))
+    s = float(Fraction(s) + Fraction(acc[3]))
+    for i in range(n4, n):
+        p = float(Fraction(x[i]) * Fraction(y[i]))
+        s = float(Fraction(s) + Fraction(p))
+    return s
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 7, 8, 13, 64, 101])
+def test_raw_dot_rounding_bit_exact(orc, n):
+    rng = np.random.default_rng(n)
+    x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    assert orc.rawDot(x, y) == _raw_dot_emulated(x, y)
+
+
+def test_matvecp_rows_are_raw_dots_and_exact_integers(orc):
+    rng = np.random.default_rng(2)
+    A, x = rng.uniform(-1, 1, (9, 37)), rng.uniform(-1, 1, 37)
+    y = orc.matvecP(A, x)
+    for i in range(9):
+        assert y[i] == _raw_dot_emulated(A[i], x)
+    Ai, xi = rng.integers(-50, 51, (20, 30)).astype(float), rng.integers(-50, 51, 30).astype(float)
+    np.testing.assert_array_equal(orc.matvecP(Ai, xi), Ai @ xi)
+
+
+def test_gemv_generic_mul_add_chain(orc):
+    rng = np.random.default_rng(3)
+    A, x, y = rng.uniform(-1, 1, (6, 11)), rng.uniform(-1, 1, 11), rng.uniform(-1, 1, 6)
+    out = orc.gemv(0.5, A, x, -2.0, y)
+    for i in range(6):
+        acc = 0.0
+        for j in range(11):
+            acc = float(Fraction(acc) + Fraction(float(Fraction(A[i, j]) * Fraction(x[j]))))
+        assert out[i] == float(Fraction(0.5 * acc) + Fraction(-2.0 * y[i]))
+
+
+def test_norms_closed_forms_and_numpy(orc):
+    A = np.array([[1.0, -2.0], [3.0, 4.0]])
+    assert orc.norm1(A) == 6.0 and orc.normInf(A) == 7.0 and orc.normFrob(A) == np.sqrt(30.0)
+    rng = np.random.default_rng(4)
+    B = rng.uniform(-1, 1, (50, 70))
+    assert abs(orc.norm1(B) - np.linalg.norm(B, 1)) <= 100 * EPS * np.linalg.norm(B, 1)
+    assert abs(orc.normInf(B) - np.linalg.norm(B, np.inf)) <= 100 * EPS * np.linalg.norm(B, np.inf)
+    assert abs(orc.normFrob(B) - np.linalg.norm(B)) <= 100 * EPS * np.linalg.norm(B)
+    # a transposed operand would swap norm1 and normInf
+    assert orc.norm1(B) != orc.normInf(B)
+
+
+def test_matmulatap_exact_and_symmetric(orc):
+    rng = np.random.default_rng(5)
+    A = rng.integers(-9, 10, (70, 45)).astype(float)
+    C = orc.matMulAtAP(A)
+    np.testing.assert_array_equal(C, A.T @ A)
+    R = rng.uniform(-1, 1, (100, 67))
+    D = orc.matMulAtAP(R)
+    np.testing.assert_array_equal(D, D.T)
+    assert np.abs(D - R.T @ R).max() <= 110 * EPS * (np.abs(R).T @ np.abs(R)).max()
+
+
+def test_det_closed_forms(orc):
+    # the reference's lu takes the later row of a tie: perm [1, 0], U = [[-1, 3], [0, 5]]
+    assert orc.det(np.array([[1.0, 2.0], [-1.0, 3.0]])) == 5.0
+    rng = np.random.default_rng(6)
+    for n in (1, 5, 9):
+        P = np.eye(n)[rng.permutation(n)]
+        assert orc.det(P) == round(np.linalg.det(P))
+    U = np.triu(rng.uniform(1, 2, (8, 8)))
+    prod = 1.0
+    for i in range(8):
+        prod = prod * U[i, i]
+    assert orc.det(U) == prod
+    H = 1.0 / (np.arange(4)[:, None] + np.arange(4)[None, :] + 1)
+    assert abs(orc.det(H) - 1.0 / 6048000.0) <= 1e-9 / 6048000.0 * 1e3
+    B = rng.uniform(-1, 1, (30, 30))
+    assert abs(orc.det(B) - np.linalg.det(B)) <= 1e-11 * abs(np.linalg.det(B))
+    assert orc.det(np.zeros((0, 0))) == 1.0
+
+
+def test_inverse_iteration_known_spectrum(orc):
+    n = 12
+    D = np.diag(np.arange(1.0, n + 1))
+    lam, q, it = orc.inverseIteration(D, 2.3, np.ones(n), 100, 1e-13)
+    assert abs(lam - 2.0) <= 1e-12 and abs(abs(q[1]) - 1.0) <= 1e-12 and it < 100
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.linspace(-3, 5, n)
+    S = (Q * ev) @ Q.T
+    S = (S + S.T) / 2
+    lam, q, it = orc.inverseIteration(S, 0.9, np.ones(n), 200, 1e-14)
+    target = ev[np.argmin(np.abs(ev - 0.9))]
+    assert abs(lam - target) <= 1e-10
+    assert np.linalg.norm(S @ q - lam * q) <= 1e-6  # the Rayleigh quotient converges first
+    lam0, q0, it0 = orc.inverseIteration(S, 0.9, np.ones(n), 0, 1e-14)
+    assert lam0 == 0.0 and it0 == 0 and np.array_equal(q0, np.ones(n))
+
+
+def test_matmulppar_partition(orc):
+    # gridDims (BLAS/Level3.hs:276-294) slices of widths divisible by 4 leave
+    # every element's arithmetic as matMulP's; exact integers agree anywhere
+    rng = np.random.default_rng(8)
+    A, B = rng.uniform(-1, 1, (256, 100)), rng.uniform(-1, 1, (100, 256))
+    np.testing.assert_array_equal(orc.matMulPPar(A, B, 8), orc.matMulP(A, B))
+    Ai, Bi = rng.integers(-9, 10, (301, 77)).astype(float), rng.integers(-9, 10, (77, 203)).astype(float)
+    for t in (1, 3, 4, 6, 16):
+        np.testing.assert_array_equal(orc.matMulPPar(Ai, Bi, t), Ai @ Bi)
