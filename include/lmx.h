@@ -115,6 +115,14 @@ int lmx_sgemm_bf16x3_dev(void *stream, char transa, char transb, int64_t m,
                          int64_t lda, const float *dB, int64_t ldb,
                          float beta, float *dC, int64_t ldc);
 
+/* The tile shape of the plain GEMMs issued by the calling thread: 0 = the
+ * autotuned table (csrc/gemm_tiles.inc, written by tune_gemm.py; shapes it
+ * does not cover take the fixed rule), 1 = 64 x 64, 2 = 128 x 64,
+ * 3 = 128 x 128.  For the autotuner, which times each shape under every
+ * tile; no reference counterpart (rawGemmKernel's dispatch,
+ * src/.../Internal/Kernel.hs:145-159, picks its path by size alone). */
+int lmx_gemm_tile_override(int tile);
+
 /* ---- LU ----------------------------------------------------------------- */
 
 /* In-place LU with partial pivoting of the n x n matrix A: unit L below the

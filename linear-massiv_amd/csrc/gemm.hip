@@ -261,11 +261,15 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
 //   for FP32 (two for FP64) instead of four scalar reads.  The row stride
 //   (20 floats, 18 doubles) puts the 128-bit reads of 8 consecutive rows on
 //   distinct banks.
-template <class T, int BM, int BN>
+//   With TB, B is stored transposed (n x k, so op(B) = B^T, as in A A^T):
+//   its k runs along its rows too, and its tiles load and store like A's.
+//   With `lower`, C is square and only the tiles on or below its diagonal
+//   are launched, and only elements on or below it are written (SYRK).
+template <class T, int BM, int BN, bool TB>
 __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alpha, const T *__restrict__ A,
                                                       int64_t lda, const T *__restrict__ B, int64_t ldb, T beta,
                                                       T *__restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
-                                                      BatchStrides bs) {
+                                                      BatchStrides bs, int lower) {
     constexpr int BK = 16;
     constexpr int SK = sizeof(T) == 4 ? BK + 4 : BK + 2;
     typedef typename V16<T>::type VT;
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
         C += zk * bs.c2 + zp * bs.c1;
     }
     int tm, tn;
-    tile_of(tiles_m, tiles_n, false, tm, tn);
+    tile_of(tiles_m, tiles_n, lower != 0, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -312,9 +316,15 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
 #pragma unroll
         for (int it = 0; it < LB; ++it) {
             const int e = it * 256 + tid;
-            const int kk = e % BK, jq = e / BK;
-            const int gj = n0 + jq * E, gk = k0 + kk;
-            rb[it] = (gk < k && gj < n) ? *reinterpret_cast<const VT *>(B + (int64_t)gk * ldb + gj) : VT{};
+            if (TB) {
+                const int j = e / (BK / E), kq = e % (BK / E);
+                const int gj = n0 + j, gk = k0 + kq * E;
+                rb[it] = (gj < n && gk < k) ? *reinterpret_cast<const VT *>(B + (int64_t)gj * ldb + gk) : VT{};
+            } else {
+                const int kk = e % BK, jq = e / BK;
+                const int gj = n0 + jq * E, gk = k0 + kk;
+                rb[it] = (gk < k && gj < n) ? *reinterpret_cast<const VT *>(B + (int64_t)gk * ldb + gj) : VT{};
+            }
         }
     };
     auto store = [&](int buf) {
@@ -327,9 +337,14 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
 #pragma unroll
         for (int it = 0; it < LB; ++it) {
             const int e = it * 256 + tid;
-            const int kk = e % BK, jq = e / BK;
+            if (TB) {
+                const int j = e / (BK / E), kq = e % (BK / E);
+                *reinterpret_cast<VT *>(&Bs[buf][j][kq * E]) = rb[it];
+            } else {
+                const int kk = e % BK, jq = e / BK;
 #pragma unroll
-            for (int q = 0; q < E; ++q) Bs[buf][jq * E + q][kk] = rb[it][q];
+                for (int q = 0; q < E; ++q) Bs[buf][jq * E + q][kk] = rb[it][q];
+            }
         }
     };
 
@@ -383,7 +398,7 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gi = m0 + wm0 + a * 16 + acc_row<T>(lk, r);
-                if (gi < m) {
+                if (gi < m && (!lower || gj <= gi)) {
                     T *c = C + (int64_t)gi * ldc + gj;
                     T v = alpha * acc[a][b][r];
                     if (beta != T(0)) v += beta * *c;
@@ -666,14 +681,16 @@ constexpr size_t gemm_nn_lds() {
     return 2 * (size_t)(BM + BN) * (sizeof(T) == 4 ? 20 : 18) * sizeof(T);
 }
 
-// Whether gemm_nn_kernel applies: no transposes, 16-byte aligned rows, k and
-// n multiples of the vector width (a vector is then wholly inside or outside).
+// Whether gemm_nn_kernel applies: A as stored, B as stored or transposed,
+// 16-byte aligned rows, and the dimensions its vectors run along (k, and n
+// for a B as stored) multiples of the vector width, so that a vector is
+// wholly inside or outside the operand.
 template <class T>
 bool nn_ok(bool ta, bool tb, int64_t n, int64_t k, const T *A, int64_t lda, const T *B, int64_t ldb,
            const BatchStrides &bs) {
     constexpr int E = V16<T>::N;
     auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
-    return !ta && !tb && al(A) && al(B) && lda % E == 0 && ldb % E == 0 && k % E == 0 && n % E == 0 &&
+    return !ta && al(A) && al(B) && lda % E == 0 && ldb % E == 0 && k % E == 0 && (tb || n % E == 0) &&
            bs.a1 % E == 0 && bs.a2 % E == 0 && bs.b1 % E == 0 && bs.b2 % E == 0;
 }
 
@@ -689,10 +706,14 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
     // kernel: the NN kernel is faster on its own but slows those panels more
     // (FP32 LU at n=16384: 155 against 151 ms).
     const bool sharing = lds_footprint > 0 && sizeof(T) == 4;
-    if (!lower && !sharing && nn_ok<T>(ta, tb, n, k, A, lda, B, ldb, bs)) {
-        const size_t st = gemm_nn_lds<T, BM, BN>();
-        hipLaunchKernelGGL((gemm_nn_kernel<T, BM, BN>), grid, block, lds_footprint > st ? lds_footprint - st : 0, s,
-                           m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m, tiles_n, bs);
+    if (!sharing && nn_ok<T>(ta, tb, n, k, A, lda, B, ldb, bs)) {
+        const size_t st = gemm_nn_lds<T, BM, BN>(), reserve = lds_footprint > st ? lds_footprint - st : 0;
+        if (tb)
+            hipLaunchKernelGGL((gemm_nn_kernel<T, BM, BN, true>), grid, block, reserve, s, m, n, k, alpha, A, lda, B,
+                               ldb, beta, C, ldc, tiles_m, tiles_n, bs, lower);
+        else
+            hipLaunchKernelGGL((gemm_nn_kernel<T, BM, BN, false>), grid, block, reserve, s, m, n, k, alpha, A, lda, B,
+                               ldb, beta, C, ldc, tiles_m, tiles_n, bs, lower);
         LMX_LAUNCH_CHECK();
         return;
     }
@@ -712,6 +733,37 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
     LMX_LAUNCH_CHECK();
 }
 
+// Tile shapes of the plain (non-batched, non-SYRK) GEMM dispatch.
+enum GemmTile { TILE_DEFAULT = 0, TILE_64x64 = 1, TILE_128x64 = 2, TILE_128x128 = 3 };
+
+// The autotuned table (gemm_tiles.inc, written by tune_gemm.py): for an
+// element size and ranges of ceil(log2) of m, n and k, the tile that timed
+// fastest on MI355X.  Shapes no rule covers take the fixed rule below.
+struct GemmTileRule {
+    int bytes;
+    int m_lo, m_hi, n_lo, n_hi, k_lo, k_hi;  // inclusive ranges of ceil(log2(dim))
+    int tile;
+};
+#include "gemm_tiles.inc"
+
+int ceil_log2(int64_t v) {
+    int b = 0;
+    while (((int64_t)1 << b) < v) ++b;
+    return b;
+}
+
+thread_local int t_tile_override = TILE_DEFAULT;
+
+int table_tile(int bytes, int64_t m, int64_t n, int64_t k) {
+    if (t_tile_override != TILE_DEFAULT) return t_tile_override;
+    const int lm = ceil_log2(m), ln = ceil_log2(n), lk = ceil_log2(std::max<int64_t>(k, 1));
+    for (const GemmTileRule &r : kGemmTileRules)
+        if (r.bytes == bytes && lm >= r.m_lo && lm <= r.m_hi && ln >= r.n_lo && ln <= r.n_hi && lk >= r.k_lo &&
+            lk <= r.k_hi)
+            return r.tile;
+    return TILE_DEFAULT;
+}
+
 template <class T>
 void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha,
                const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, size_t lds_footprint = 0,
@@ -719,10 +771,29 @@ void gemm_impl(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
     if (m < 0 || n < 0 || k < 0) fail_arg("gemm: negative dimension");
     if (m == 0 || n == 0) return;
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("gemm: dimension too large");
-    // Large outputs: 128 x 128 tiles.  Smaller ones: 64 x 64 tiles so the
-    // launch still spreads over the 256 CUs.  Each tile size is its own
-    // timing family (one kernel each).
     const double flops = 2.0 * (double)m * (double)n * (double)k;
+    // A GEMM sharing its CUs with the LU panels keeps its measured tile (the
+    // table is timed on an idle device); the others take the tuned table's
+    // tile where it has one.
+    const int tile = sharing ? TILE_DEFAULT : table_tile((int)sizeof(T), m, n, k);
+    if (tile != TILE_DEFAULT) {
+        const BatchStrides nob{1, 0, 0, 0, 0, 0, 0};
+        const bool big = tile != TILE_64x64;
+        TimedRegion tr(sizeof(T) == 8 ? (big ? "gemm" : "gemm64") : (big ? "sgemm" : "sgemm64"), s, flops);
+        if (tile == TILE_64x64)
+            launch_tile<T, 64, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 1,
+                                       nob, lds_footprint);
+        else if (tile == TILE_128x64)
+            launch_tile<T, 128, 64, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 1,
+                                        nob, lds_footprint);
+        else
+            launch_tile<T, 128, 128, 16>(s, ta, tb, (int)m, (int)n, (int)k, alpha, A, lda, B, ldb, beta, C, ldc, 0,
+                                         1, nob, lds_footprint);
+        return;
+    }
+    // The fixed rule: large outputs take 128 x 128 tiles, smaller ones
+    // 64 x 64 so the launch still spreads over the 256 CUs.  Each tile size
+    // is its own timing family.
     int64_t big_tiles = cdiv(m, 128) * cdiv(n, 128);
     if (big_tiles >= 512) {
         TimedRegion tr(sizeof(T) == 8 ? "gemm" : "sgemm", s, flops);
@@ -922,6 +993,14 @@ int lmx_sgemm_dev(void *stream, char transa, char transb, int64_t m, int64_t n, 
         bool ta = is_trans(transa), tb = is_trans(transb);
         check_gemm_args(ta, tb, m, n, k, lda, ldb, ldc);
         gemm(resolve_stream(stream), ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc);
+        return LMX_OK;
+    });
+}
+
+int lmx_gemm_tile_override(int tile) {
+    return guarded([&] {
+        if (tile < TILE_DEFAULT || tile > TILE_128x128) fail_arg("gemm_tile_override: tile must be 0..3");
+        t_tile_override = tile;
         return LMX_OK;
     });
 }

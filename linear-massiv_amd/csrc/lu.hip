@@ -1071,20 +1071,21 @@ void clear_info(hipStream_t s, int32_t *d_info) {
 // getrf with optional permutation-vector output (d_perm may be null).
 // A22 -= L21 U12 for the trailing updates: FP32 or FP64 MFMA tiles, or, for
 // the BF16 factor (bf16 = true, FP32 storage), the BF16 tiles with each
-// operand split in two BF16 parts.  A nonzero lds_footprint holds the
-// workgroups per CU down so that panel workgroups find room beside them.
+// operand split in two BF16 parts.  `sharing`: the lookahead GEMM that runs
+// beside the panels, with the tiles and LDS footprint measured for that
+// (its workgroups per CU held down so that panel workgroups find room).
 template <class T>
 void trailing_update(hipStream_t s, bool bf16, int64_t m, int64_t n, int64_t k, const T *L21, int64_t lda,
-                     const T *U12, T *A22, size_t lds_footprint) {
+                     const T *U12, T *A22, bool sharing) {
     if constexpr (sizeof(T) == 4) {
         if (bf16) {
             gemm_bf16x3(s, false, false, m, n, k, -1.0f, L21, lda, U12, lda, 1.0f, A22, lda, false,
-                        lds_footprint ? BF16_SHARING_LDS : 0);
+                        sharing ? BF16_SHARING_LDS : 0);
             return;
         }
     }
-    if (lds_footprint)
-        gemm_sharing<T>(s, m, n, k, T(-1), L21, lda, U12, lda, T(1), A22, lda, lds_footprint);
+    if (sharing)
+        gemm_sharing<T>(s, m, n, k, T(-1), L21, lda, U12, lda, T(1), A22, lda, aux_gemm_lds_footprint<T>());
     else
         gemm(s, false, false, m, n, k, T(-1), L21, lda, U12, lda, T(1), A22, lda);
 }
@@ -1189,7 +1190,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
             tri_inverse_solve<T>(s, true, false, nbw, narrow, A11, lda, W, A12, lda);
             trailing_update<T>(s, bf16, trail, narrow, nbw, A11 + (int64_t)nbw * lda, lda, A12,
-                               A12 + (int64_t)nbw * lda, 0);
+                               A12 + (int64_t)nbw * lda, false);
         }
         LMX_HIP(hipEventRecord(evE, s));
         LMX_HIP(hipStreamWaitEvent(aux, evE, 0));
@@ -1199,7 +1200,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             // at most two GEMM workgroups per CU, so every CU keeps room for
             // a panel workgroup of the caller's stream
             trailing_update<T>(aux, bf16, trail, trail - narrow, nbw, A11 + (int64_t)nbw * lda, lda, A12 + narrow,
-                               A12 + (int64_t)nbw * lda + narrow, aux_gemm_lds_footprint<T>());
+                               A12 + (int64_t)nbw * lda + narrow, true);
         }
         LMX_HIP(hipEventRecord(evF, aux));
         pending = true;

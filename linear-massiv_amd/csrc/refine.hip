@@ -59,9 +59,9 @@ __global__ void eta_kernel(double *norms, int k, double crit) {
 // Givens rotations.  Outer steps recompute r = b - A x in FP64 and test
 // convergence with the same rule as plain refinement.
 constexpr int GM_COLS = 64;      // right-hand sides per Krylov group
-constexpr int GM_RESTART = 40;   // basis vectors per outer step
+constexpr int GM_RESTART = 50;   // basis vectors per outer step
 constexpr int GM_OUTER = 12;     // outer steps
-constexpr double GM_INNER_TOL = 1e-10;
+constexpr double GM_INNER_TOL = 1e-8;
 constexpr int GM_CHUNK = 1024;   // rows per partial dot product
 
 // part[(i * nchunk + chunk) * k + c] = sum over the chunk's rows of

@@ -58,7 +58,7 @@ EXPORTED = [
     "lmx_dgemm", "lmx_dgemm_dev", "lmx_sgemm", "lmx_sgemm_dev",
     "lmx_dgetrf", "lmx_dgetrf_dev", "lmx_sgetrf", "lmx_sgetrf_dev", "lmx_dgetrs", "lmx_dgetrs_dev", "lmx_dgesv", "lmx_dgesv_dev",
     "lmx_dlu", "lmx_dlu_dev", "lmx_slu", "lmx_slu_dev",
-    "lmx_gemm_bf16", "lmx_gemm_bf16_dev", "lmx_sgemm_bf16x3_dev",
+    "lmx_gemm_bf16", "lmx_gemm_bf16_dev", "lmx_sgemm_bf16x3_dev", "lmx_gemm_tile_override",
     "lmx_dgemv", "lmx_dgemv_dev", "lmx_dnorm", "lmx_dnorm_dev", "lmx_dresidual_dev", "lmx_dscaled_residual",
     "lmx_dscaled_residual_dev", "lmx_dsyrk", "lmx_dsyrk_dev", "lmx_ddet", "lmx_ddet_dev",
     "lmx_dinverse_iteration", "lmx_dinverse_iteration_dev",
@@ -135,6 +135,7 @@ _sigs = {
     "lmx_gemm_bf16": [_chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
     "lmx_gemm_bf16_dev": [_vp, _chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
     "lmx_sgemm_bf16x3_dev": [_vp, _chr, _chr, _i64, _i64, _i64, _flt, _vp, _i64, _vp, _i64, _flt, _vp, _i64],
+    "lmx_gemm_tile_override": [ctypes.c_int],
     "lmx_dgemv": [_i64, _i64, _dbl, _vp, _i64, _vp, _dbl, _vp],
     "lmx_dgemv_dev": [_vp, _i64, _i64, _dbl, _vp, _i64, _vp, _dbl, _vp],
     "lmx_dnorm": [_chr, _i64, _i64, _vp, _i64, ctypes.POINTER(_dbl)],
@@ -805,6 +806,12 @@ def sgemm_bf16x3_dev(A, B, C, alpha=1.0, beta=0.0, transa="N", transb="N", strea
     m, n, k = _gemm_shapes(A, B, C, transa, transb, "f32")
     _check(_lib.lmx_sgemm_bf16x3_dev(_stream(stream), transa.encode(), transb.encode(), m, n, k, alpha, _tp(A),
                                      _ld(A), _tp(B), _ld(B), beta, _tp(C), _ld(C)))
+
+
+def gemm_tile_override(tile):
+    """Forces the calling thread's plain GEMMs onto one tile shape (1 = 64x64,
+    2 = 128x64, 3 = 128x128); 0 returns to the tuned table."""
+    _check(_lib.lmx_gemm_tile_override(tile))
 
 
 def getrf_dev(A, ipiv, info, stream=None):

@@ -111,3 +111,26 @@ def test_large_tile_paths_exact(lmx_gpu, m, n, k):
     exact = A @ B - C0  # integer-valued, |entries| < 2^24: exact in float64 and float32
     np.testing.assert_array_equal(lmx_gpu.dgemm(A, B, C0, alpha=1.0, beta=-1.0), exact)
     np.testing.assert_array_equal(lmx_gpu.sgemm(A, B, C0, alpha=1.0, beta=-1.0).astype(float), exact)
+
+
+@pytest.mark.parametrize("m,n,k", [(3000, 3000, 520), (200, 300, 64), (3000, 2999, 516)])
+def test_nt_kernel_exact(lmx_gpu, m, n, k):
+    # op(B) = B^T with B's rows 16-byte aligned: the NN kernel loads B's tiles
+    # along k like A's (128 and 64 tiles; n need not be a multiple of 4)
+    rng = np.random.default_rng(m + 2 * n + k)
+    A = rng.integers(-8, 9, (m, k)).astype(float)
+    Bt = rng.integers(-8, 9, (n, k)).astype(float)
+    C0 = rng.integers(-8, 9, (m, n)).astype(float)
+    exact = A @ Bt.T + 2 * C0
+    np.testing.assert_array_equal(lmx_gpu.dgemm(A, Bt, C0, alpha=1.0, beta=2.0, transb="T"), exact)
+    np.testing.assert_array_equal(lmx_gpu.sgemm(A, Bt, C0, alpha=1.0, beta=2.0, transb="T").astype(float), exact)
+
+
+@pytest.mark.parametrize("n,k", [(4100, 520), (1000, 64), (333, 36)])
+def test_syrk_lower_tiles_exact(lmx_gpu, n, k):
+    # A A^T on lower tiles only (the Cholesky update), mirrored by lmx_dsyrk
+    rng = np.random.default_rng(n + k)
+    A = rng.integers(-8, 9, (n, k)).astype(float)
+    C0 = rng.integers(-8, 9, (n, n)).astype(float)
+    C0 = C0 + C0.T
+    np.testing.assert_array_equal(lmx_gpu.syrk(A, "N", alpha=-1.0, beta=1.0, c=C0), C0 - A @ A.T)
