@@ -246,6 +246,17 @@ int lmx_dscaled_residual_dev(void *stream, int64_t n, int64_t nrhs,
                              const double *dA, int64_t lda, const double *dX,
                              int64_t ldx, const double *dB, int64_t ldb,
                              double *d_eta);
+/* lmx_dscaled_residual_dev without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Its accumulators live in a
+ * fixed device array of the library (64 doubles per GPU, kept until the
+ * process ends), so nrhs is limited to 16 (larger values return
+ * LMX_ERR_ARG).  Calls that overlap on one GPU share that array and race. */
+int lmx_dscaled_residual_noalloc_dev(void *stream, int64_t n, int64_t nrhs,
+                                     const double *dA, int64_t lda,
+                                     const double *dX, int64_t ldx,
+                                     const double *dB, int64_t ldb,
+                                     double *d_eta);
 /* C = alpha op(A) op(A)^T + beta C, the whole symmetric n x n C (its lower
  * triangle computed on the FP64 MFMA tiles, then mirrored).  trans 'N':
  * op(A) = A (n x k); 'T': op(A) = A^T with A stored k x n, so alpha = 1,

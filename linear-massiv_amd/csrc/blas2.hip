@@ -891,3 +891,57 @@ int lmx_dinverse_iteration(int64_t n, const double *A, int64_t lda, double mu, c
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- without allocation
+//
+// lmx_dscaled_residual_noalloc_dev keeps its accumulators (||A||_inf and,
+// per column, ||b||, ||x||, ||r||) in a file-scope device array instead of
+// memory from the pool, for callers that cannot allocate.  Each GPU holds
+// its own copy of the array for the life of the process; its address is
+// looked up once per device.  Every call clears what it uses first; calls
+// that overlap on one GPU share the array and race.
+namespace lmx {
+namespace {
+
+constexpr int NOALLOC_MAX_NRHS = 16;  // the fused residual path: no residual block
+constexpr int NOALLOC_DOUBLES = 64;   // >= 1 + 3 NOALLOC_MAX_NRHS
+static_assert(NOALLOC_DOUBLES >= 1 + 3 * NOALLOC_MAX_NRHS, "scratch too small");
+static_assert(NOALLOC_MAX_NRHS <= SCALED_RESIDUAL_FUSED_MAX, "the no-allocation path stores no residual");
+
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64] = {};
+
+// The device address of `scratch` on the current device.
+double *scratch_on_device() {
+    int dev = 0;
+    LMX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error(LMX_ERR_DEVICE, "device id out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        LMX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+}  // namespace lmx
+
+extern "C" int lmx_dscaled_residual_noalloc_dev(void *stream, int64_t n, int64_t nrhs, const double *dA,
+                                                int64_t lda, const double *dX, int64_t ldx, const double *dB,
+                                                int64_t ldb, double *d_eta) {
+    return guarded([&] {
+        check_mat(n, n, lda, "scaled_residual_noalloc");
+        check_mat(n, nrhs, ldx, "scaled_residual_noalloc");
+        check_mat(n, nrhs, ldb, "scaled_residual_noalloc");
+        if (nrhs > NOALLOC_MAX_NRHS)
+            fail_arg("scaled_residual_noalloc: nrhs above 16, the columns its device scratch holds");
+        if (nrhs == 0) return LMX_OK;
+        double *acc = scratch_on_device();
+        scaled_residual(resolve_stream(stream), n, nrhs, dA, lda, dX, ldx, dB, ldb, d_eta, acc, nullptr);
+        return LMX_OK;
+    });
+}

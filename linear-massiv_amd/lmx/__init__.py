@@ -60,7 +60,7 @@ EXPORTED = [
     "lmx_dlu", "lmx_dlu_dev", "lmx_slu", "lmx_slu_dev",
     "lmx_gemm_bf16", "lmx_gemm_bf16_dev", "lmx_sgemm_bf16x3_dev", "lmx_gemm_tile_override",
     "lmx_dgemv", "lmx_dgemv_dev", "lmx_dnorm", "lmx_dnorm_dev", "lmx_dresidual_dev", "lmx_dscaled_residual",
-    "lmx_dscaled_residual_dev", "lmx_dsyrk", "lmx_dsyrk_dev", "lmx_ddet", "lmx_ddet_dev",
+    "lmx_dscaled_residual_dev", "lmx_dscaled_residual_noalloc_dev", "lmx_dsyrk", "lmx_dsyrk_dev", "lmx_ddet", "lmx_ddet_dev",
     "lmx_dinverse_iteration", "lmx_dinverse_iteration_dev",
     "lmx_factor_create", "lmx_factor_create_dev", "lmx_factor_solve", "lmx_factor_solve_dev", "lmx_factor_query",
     "lmx_factor_release",
