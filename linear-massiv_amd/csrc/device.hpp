@@ -11,6 +11,7 @@ namespace lmx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 template <class T>
 struct V4;
 template <>

@@ -342,6 +342,18 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     }
 }
 
+// 16-byte vectors of panel row elements.
+template <class T>
+struct RowVec;
+template <>
+struct RowVec<float> {
+    typedef f4 type;
+};
+template <>
+struct RowVec<double> {
+    typedef d2 type;
+};
+
 template <class T, bool TL>
 __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
                                                           int k0, int32_t *__restrict__ ipiv,
@@ -354,13 +366,29 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
     const int w = blockIdx.x, t = threadIdx.x;
     __shared__ PanelLds<T> L;
     PanelRows<T> st;
+    // A full-width panel whose rows start on 16-byte boundaries moves each
+    // row in 16-byte vectors (a quarter of the address work of 4-byte ones)
+    typedef typename RowVec<T>::type V;
+    constexpr int VE = 16 / sizeof(T);
+    const bool vec = kb == KB && (reinterpret_cast<uintptr_t>(A) & 15) == 0 && ((lda * (int64_t)sizeof(T)) & 15) == 0;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         st.row[q] = w * PANEL_ROWS + q * PT + t;
         st.tag[q] = st.row[q];
+        if (vec) {
+            const T *src = A + (int64_t)st.row[q] * lda;
 #pragma unroll
-        for (int c = 0; c < KB; ++c)
-            st.v[q][c] = (st.row[q] < m && c < kb) ? A[(int64_t)st.row[q] * lda + c] : T(0);
+            for (int c = 0; c < KB; c += VE) {
+                V x = {};
+                if (st.row[q] < m) x = *reinterpret_cast<const V *>(src + c);
+#pragma unroll
+                for (int e = 0; e < VE; ++e) st.v[q][c + e] = x[e];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < KB; ++c)
+                st.v[q][c] = (st.row[q] < m && c < kb) ? A[(int64_t)st.row[q] * lda + c] : T(0);
+        }
     }
     int local_info = 0;
     bool failed = false;
@@ -385,30 +413,48 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
     }
     __syncthreads();
 
-    // After the rotations register s holds column (s + PG*groups) % KB.
+    // After the rotations register s holds column (s + PG*groups) % KB
+    // (column s itself for a full-width panel: KB / PG rotations).
     const int shift = (PG * groups) % KB;
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
         if (st.row[q] < m) {
+            T *dst = A + (int64_t)st.row[q] * lda;
+            if (vec) {
 #pragma unroll
-            for (int s2 = 0; s2 < KB; ++s2) {
-                int c = (s2 + shift) % KB;
-                if (c < kb) A[(int64_t)st.row[q] * lda + c] = st.v[q][s2];
+                for (int c = 0; c < KB; c += VE) {
+                    V x;
+#pragma unroll
+                    for (int e = 0; e < VE; ++e) x[e] = st.v[q][c + e];
+                    *reinterpret_cast<V *>(dst + c) = x;
+                }
+            } else {
+#pragma unroll
+                for (int s2 = 0; s2 < KB; ++s2) {
+                    int c = (s2 + shift) % KB;
+                    if (c < kb) dst[c] = st.v[q][s2];
+                }
             }
             srcmap[st.row[q]] = st.tag[q];
         }
-    if (w == 0 && t == 0) {
-        int cnt = 0;
-        for (int d = 0; d < kb && d < m; ++d) moves[1 + cnt++] = d;
-        for (int j = 0; j < kb; ++j) {
-            int pj = L.pivs[j];
-            if (pj < kb) continue;
-            bool dup = false;
-            for (int x = 0; x < j; ++x) dup |= (L.pivs[x] == pj);
-            if (!dup) moves[1 + cnt++] = pj;
+    // The row-move list: panel rows 0 .. min(kb, m) - 1, then every pivot
+    // row below the panel's diagonal block, once, in the order of the
+    // columns that first chose it.  Lane j of wave 0 takes column j's pivot
+    // and a ballot gives its place in the list.
+    if (w == 0 && t < 64) {
+        const int nd = min(kb, m);
+        if (t < nd) moves[1 + t] = t;
+        const int pj = t < kb ? L.pivs[t] : -1;
+        bool keep = t < kb && pj >= kb;
+#pragma unroll
+        for (int x = 0; x < KB; ++x)
+            if (x < kb) keep = keep && !(x < t && L.pivs[x] == pj);
+        const unsigned long long mask = __ballot(keep);
+        if (keep) moves[1 + nd + __popcll(mask & ((1ull << t) - 1ull))] = pj;
+        if (t == 0) {
+            moves[0] = nd + __popcll(mask);
+            if (local_info != 0 && *info == 0) *info = local_info;
         }
-        moves[0] = cnt;
-        if (local_info != 0 && *info == 0) *info = local_info;
     }
 }
 
