@@ -149,59 +149,59 @@ __device__ __forceinline__ void wg_select(const T *vals, const int *ids, T &best
     }
 }
 
+// The tie rule's bit among the set bits of a wave-uniform mask whose bit
+// order is row order: the lowest (rawPivotSearch) or the highest (lu's
+// maximumBy); -1 for an empty mask.
+template <bool TL>
+__device__ __forceinline__ int tie_index(unsigned long long mask) {
+    if (mask == 0) return -1;
+    return TL ? 63 - __builtin_clzll(mask) : __builtin_ctzll(mask);
+}
+
 // One column (j = j0 + U) of the panel: pivot search, interchange, scaling
-// and rank-1 update of the rest of the panel.  Returns the pivot row.
+// and rank-1 update of the rest of the panel.
 template <class T, int U, bool TL>
 __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, int j0, int kb, int m, int P,
                                              PanelSync<T> *__restrict__ sy, unsigned base, int k0,
                                              int32_t *__restrict__ ipiv, int &local_info, bool &failed) {
+    static_assert(RPT == 1, "one panel row per thread");
     typedef PanelRec<T> R;
     typedef TieRule<TL> Tie;
     const int j = j0 + U;
     if (j >= kb) return;
     const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int par = j & 1;
-    // 1. this wave's candidate: max |a_ij| over its rows i >= j, then the
-    //    tie rule's row attaining it; its owner writes the row to the wave slot
-    T a[RPT];
-    T tmax = T(-1);
+    const int row = st.row[0];
+    const bool live = row >= j && row < m;
+    // 1. this wave's candidate: max |a_ij| over its rows i >= j (fmax skips
+    //    NaN), then the tie rule's lane among the rows attaining it (a
+    //    wave's lanes hold consecutive rows, so lane order is row order);
+    //    that lane writes its row to the wave slot
+    const T a = live ? fabs(st.v[0][U]) : T(-1);
+    const T wmax = wave_max(a);
+    const int wl = tie_index<TL>(__ballot(a == wmax && a >= T(0)));
+    if (lane == wl) {
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        a[q] = (st.row[q] >= j && st.row[q] < m) ? fabs(st.v[q][U]) : T(-1);
-        tmax = fmax(tmax, a[q]);
+        for (int c = 0; c < KB; ++c) L.slot_data[par][wave][c] = st.v[0][c];
+        L.slot_tag[par][wave] = st.tag[0];
     }
-    const T wmax = wave_max(tmax);
-    int trow = Tie::NONE;
+    if (row == j) {
 #pragma unroll
-    for (int q = 0; q < RPT; ++q)
-        if (a[q] == wmax && a[q] >= T(0)) trow = Tie::pick(trow, st.row[q]);
-    const int wrow = Tie::wave_pick(trow);
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        if (st.row[q] == wrow) {
-#pragma unroll
-            for (int c = 0; c < KB; ++c) L.slot_data[par][wave][c] = st.v[q][c];
-            L.slot_tag[par][wave] = st.tag[q];
-        }
-        if (st.row[q] == j) {
-#pragma unroll
-            for (int c = 0; c < KB; ++c) L.drow[par][c] = st.v[q][c];
-            L.dtag[par] = st.tag[q];
-        }
+        for (int c = 0; c < KB; ++c) L.drow[par][c] = st.v[0][c];
+        L.dtag[par] = st.tag[0];
     }
     if (lane == 0) {
-        L.slot_val[par][wave] = wmax;
-        L.slot_row[par][wave] = wrow;
+        // a wave with no candidate offers -1, so no slot holds a NaN
+        L.slot_val[par][wave] = wl < 0 ? T(-1) : wmax;
+        L.slot_row[par][wave] = wl < 0 ? Tie::NONE : w * PANEL_ROWS + wave * 64 + wl;
     }
     __syncthreads();
-    // 2. this workgroup's candidate
-    T best;
-    int bw, brow;
-    wg_select<T, TL>(L.slot_val[par], L.slot_row[par], best, brow, bw);
-    if (!(best >= T(0))) {
-        bw = -1;
-        brow = Tie::NONE;
-    }
+    // 2. this workgroup's candidate: lane x reads slot x; waves hold
+    //    ascending rows, so the tie rule over rows is the one over waves
+    const T sv = lane < PW ? L.slot_val[par][lane] : T(-1);
+    const T best = wave_max(sv);
+    const int bw = tie_index<TL>(__ballot(lane < PW && sv == best && best >= T(0)));
+    const int brow = bw < 0 ? Tie::NONE : L.slot_row[par][bw];
     int p, ptag, dtag;
     const T *prow;
     const T *drow;
@@ -220,30 +220,26 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
         }
     } else {
         const unsigned tg = base + (unsigned)j + 1u;
-        // 3. publish this workgroup's record (and row j from workgroup 0)
+        // 3. publish this workgroup's record (and row j from workgroup 0):
+        //    lane e of wave 0 writes word e
         if (wave == 0) {
             unsigned long long *rec = sy->rec[par] + w * R::W;
+            const unsigned *data = reinterpret_cast<const unsigned *>(L.slot_data[par][bw < 0 ? 0 : bw]);
+            const unsigned btag = bw < 0 ? 0u : (unsigned)L.slot_tag[par][bw];
             for (int e = lane; e < R::W; e += 64) {
-                unsigned pl = 0;
-                if (e < R::WPV)
-                    pl = word_of(best, e);
-                else if (e == R::WPV)
-                    pl = (unsigned)brow;
-                else if (e == R::WPV + 1)
-                    pl = bw >= 0 ? (unsigned)L.slot_tag[par][bw] : 0u;
-                else if (e >= R::DATA)
-                    pl = bw >= 0 ? word_of(L.slot_data[par][bw][(e - R::DATA) / R::WPV], (e - R::DATA) % R::WPV) : 0u;
+                const unsigned dv = data[max(e - R::DATA, 0)];
+                const unsigned hv = e < R::WPV ? word_of(best, e) : (e == R::WPV ? (unsigned)brow : btag);
+                const unsigned pl = e >= R::DATA ? (bw < 0 ? 0u : dv) : hv;
                 xstore(rec + e, tagword(pl, tg));
             }
-            if (w == 0)
+            if (w == 0) {
+                const unsigned *dd = reinterpret_cast<const unsigned *>(L.drow[par]);
                 for (int e = lane; e < R::DW; e += 64) {
-                    unsigned pl = 0;
-                    if (e == 0)
-                        pl = (unsigned)L.dtag[par];
-                    else if (e >= R::DDATA)
-                        pl = word_of(L.drow[par][(e - R::DDATA) / R::WPV], (e - R::DDATA) % R::WPV);
+                    const unsigned dv = dd[max(e - R::DDATA, 0)];
+                    const unsigned pl = e >= R::DDATA ? dv : (e == 0 ? (unsigned)L.dtag[par] : 0u);
                     xstore(sy->drow[par] + e, tagword(pl, tg));
                 }
+            }
         }
         // 4. poll every record: all loads in flight first, then the
         //    stragglers one by one
@@ -278,17 +274,14 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
             }
         }
         __syncthreads();
-        // 5. the pivot: largest value over the workgroups, the tie rule's
-        //    row on ties; every wave reduces the P records itself (lane x
-        //    reads record x), so no serial loop and no barrier
+        // 5. the pivot: largest value over the workgroups; workgroups hold
+        //    ascending rows, so the tie rule over rows is the one over
+        //    workgroups.  Every wave reduces the P records itself (lane x
+        //    reads record x).  Record values are -1 or a magnitude, never NaN.
         const unsigned *pay = L.pay[par];
         const T xv = lane < P ? *reinterpret_cast<const T *>(pay + lane * R::W) : T(-1);
-        const int xr = lane < P ? (int)pay[lane * R::W + R::WPV] : Tie::NONE;
         const T gb = wave_max(xv);
-        const int gr = Tie::wave_pick((gb >= T(0) && xv == gb) ? xr : Tie::NONE);
-        // the workgroup that owns the pivot row (rows are split in
-        // PANEL_ROWS-row blocks)
-        const int gw = gr == Tie::NONE ? -1 : gr / PANEL_ROWS;
+        const int gw = tie_index<TL>(__ballot(lane < P && xv == gb && gb >= T(0)));
         drow = reinterpret_cast<const T *>(pay + MAXP * R::W + R::DDATA);
         dtag = (int)pay[MAXP * R::W];
         if (gw < 0 || isnan(drow[U])) {
@@ -296,47 +289,47 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
             prow = drow;
             ptag = dtag;
         } else {
-            p = gr;
+            p = (int)pay[gw * R::W + R::WPV];
             prow = reinterpret_cast<const T *>(pay + gw * R::W + R::DATA);
             ptag = (int)pay[gw * R::W + R::WPV + 1];
         }
     }
     // 6. interchange rows j and p
     if (p != j) {
+        if (row == p) {
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            if (st.row[q] == p) {
+            for (int c = 0; c < KB; ++c) st.v[0][c] = drow[c];
+            st.tag[0] = dtag;
+        } else if (row == j) {
 #pragma unroll
-                for (int c = 0; c < KB; ++c) st.v[q][c] = drow[c];
-                st.tag[q] = dtag;
-            } else if (st.row[q] == j) {
-#pragma unroll
-                for (int c = 0; c < KB; ++c) st.v[q][c] = prow[c];
-                st.tag[q] = ptag;
-            }
+            for (int c = 0; c < KB; ++c) st.v[0][c] = prow[c];
+            st.tag[0] = ptag;
         }
     }
     if (t == 0) L.pivs[j] = p;
     if (w == 0 && t == 0) ipiv[k0 + j] = k0 + p;
     // 7. multipliers (a division, as the reference) and the rank-1 update
-    //    of the panel columns after j
-    // the pivot row is read from LDS in one batch first: a read under the
-    // per-column guard below would wait for each load in turn
+    //    of the panel columns after j.  The pivot row is read from LDS in
+    //    one batch first.  Registers past KB - j0 hold the finished columns
+    //    (rotated to the end) and are skipped in blocks of PG; registers of
+    //    columns past kb (a narrow last panel) are padding that is updated
+    //    but never stored.
     T pr[KB];
 #pragma unroll
     for (int c = U; c < KB; ++c) pr[c] = prow[c];
     const T piv = pr[U];
-    const int lim = kb - j0;
     if (piv != T(0)) {
+        if (live && row > j) {
+            const T l = st.v[0][U] / piv;
+            st.v[0][U] = l;
 #pragma unroll
-        for (int q = 0; q < RPT; ++q)
-            if (st.row[q] > j && st.row[q] < m) {
-                const T l = st.v[q][U] / piv;
-                st.v[q][U] = l;
+            for (int b = (U + 1) / PG * PG; b < KB; b += PG)
+                if (b < KB - j0) {
 #pragma unroll
-                for (int c = U + 1; c < KB; ++c)
-                    if (c < lim) st.v[q][c] = fma(-l, pr[c], st.v[q][c]);
-            }
+                    for (int e = 0; e < PG; ++e)
+                        if (b + e > U) st.v[0][b + e] = fma(-l, pr[b + e], st.v[0][b + e]);
+                }
+        }
     } else if (local_info == 0) {
         local_info = k0 + j + 1;
     }
