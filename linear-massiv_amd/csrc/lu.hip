@@ -112,6 +112,9 @@ struct PanelLds {
     int dtag[2];
     int pivs[KB];
     alignas(8) unsigned pay[2][PanelRec<T>::TOTAL];  // polled record payloads (several workgroups)
+    int mv_dst[2 * KB];  // the rows that move (panel-relative) ...
+    int mv_src[2 * KB];  // ... and the original row each one receives
+    int mv_cnt;
 };
 
 // Tie rules of the pivot search: the lowest row among equal magnitudes
@@ -347,14 +350,25 @@ struct RowVec<double> {
     typedef d2 type;
 };
 
+// The row interchanges of a panel applied to other columns of the matrix
+// (rows relative to the panel's first row): columns [c0, c1) and [c2, c3)
+// of G, and the permutation vector perm when not null; pairs_out receives
+// the move list (rows absolute) for the columns left for later.
+template <class T>
+struct PanelGather {
+    T *G;
+    int64_t ldg;
+    int c0, c1, c2, c3;
+    int32_t *perm;
+    int32_t *pairs_out;
+};
+
 template <class T, bool TL>
 __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
                                                           int k0, int32_t *__restrict__ ipiv,
-                                                          int32_t *__restrict__ srcmap,
-                                                          int32_t *__restrict__ moves,
                                                           int32_t *__restrict__ info,
                                                           PanelSync<T> *__restrict__ sy, int P,
-                                                          unsigned base) {
+                                                          unsigned base, PanelGather<T> gx) {
     critical_path_priority();
     const int w = blockIdx.x, t = threadIdx.x;
     __shared__ PanelLds<T> L;
@@ -428,26 +442,77 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
                     if (c < kb) dst[c] = st.v[q][s2];
                 }
             }
-            srcmap[st.row[q]] = st.tag[q];
         }
-    // The row-move list: panel rows 0 .. min(kb, m) - 1, then every pivot
-    // row below the panel's diagonal block, once, in the order of the
-    // columns that first chose it.  Lane j of wave 0 takes column j's pivot
-    // and a ballot gives its place in the list.
-    if (w == 0 && t < 64) {
-        const int nd = min(kb, m);
-        if (t < nd) moves[1 + t] = t;
+    // The row interchanges on the other columns.  Every workgroup knows
+    // every pivot (L.pivs), so each one derives the move list itself:
+    //  a. the rows that move: panel rows 0 .. min(kb, m) - 1, then each pivot
+    //     row below the panel's diagonal block once, in the order of the
+    //     columns that first chose it (lane j of wave 0 takes column j's
+    //     pivot; a ballot gives its place in the list);
+    //  b. the original row each of them receives: lane e holds the tag of
+    //     the row at position mv_dst[e] and the kb interchanges are
+    //     replayed on those tags;
+    //  c. the gather itself, 128 columns a pass, each column read by one
+    //     workgroup before it is written.
+    const int nd = min(kb, m);
+    if (t < 64) {
         const int pj = t < kb ? L.pivs[t] : -1;
         bool keep = t < kb && pj >= kb;
 #pragma unroll
         for (int x = 0; x < KB; ++x)
             if (x < kb) keep = keep && !(x < t && L.pivs[x] == pj);
         const unsigned long long mask = __ballot(keep);
-        if (keep) moves[1 + nd + __popcll(mask & ((1ull << t) - 1ull))] = pj;
-        if (t == 0) {
-            moves[0] = nd + __popcll(mask);
-            if (local_info != 0 && *info == 0) *info = local_info;
+        if (t < nd) L.mv_dst[t] = t;
+        if (keep) L.mv_dst[nd + __popcll(mask & ((1ull << t) - 1ull))] = pj;
+        if (t == 0) L.mv_cnt = nd + __popcll(mask);
+    }
+    __syncthreads();
+    const int cnt = L.mv_cnt;
+    if (t < 64) {
+        const int pos = t < cnt ? L.mv_dst[t] : -1;
+        const int pv = t < kb ? L.pivs[t] : 0;
+        int tag = pos;
+        for (int jj = 0; jj < kb; ++jj) {
+            const int p = __builtin_amdgcn_readlane(pv, jj);
+            if (p == jj) continue;
+            // the lane holding position p (positions below kb are lanes)
+            const int lp = p < nd ? p : __builtin_ctzll(__ballot(pos == p));
+            const int tj = __builtin_amdgcn_readlane(tag, jj);
+            const int tp = __builtin_amdgcn_readlane(tag, lp);
+            tag = t == jj ? tp : (t == lp ? tj : tag);
         }
+        L.mv_src[t] = t < cnt ? tag : 0;
+        if (t >= cnt) L.mv_dst[t] = 0;
+        if (w == 0 && t == 0 && local_info != 0 && *info == 0) *info = local_info;
+    }
+    __syncthreads();
+    if (w == 0 && gx.pairs_out != nullptr && t < 64) {
+        if (t == 0) gx.pairs_out[0] = cnt;
+        gx.pairs_out[1 + t] = k0 + L.mv_dst[t];
+        gx.pairs_out[1 + 2 * KB + t] = k0 + L.mv_src[t];
+    }
+    const int n1 = gx.c1 - gx.c0, ncols = n1 + (gx.c3 - gx.c2);
+    constexpr int GCOLS = 128, GPER = 2 * KB / (PT / GCOLS);  // columns a pass; moves per thread
+    for (int cb = w * GCOLS; cb < ncols; cb += P * GCOLS) {
+        const int cc = cb + (t % GCOLS), grp = t / GCOLS;
+        const int col = cc < n1 ? gx.c0 + cc : (cc < ncols ? gx.c2 + (cc - n1) : -1);
+        T val[GPER];
+#pragma unroll
+        for (int i = 0; i < GPER; ++i) {
+            const int e = grp * GPER + i;
+            val[i] = (col >= 0 && e < cnt) ? gx.G[(int64_t)L.mv_src[e] * gx.ldg + col] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < GPER; ++i) {
+            const int e = grp * GPER + i;
+            if (col >= 0 && e < cnt) gx.G[(int64_t)L.mv_dst[e] * gx.ldg + col] = val[i];
+        }
+    }
+    if (w == 0 && gx.perm != nullptr) {
+        const int pv = t < cnt ? gx.perm[L.mv_src[t]] : 0;
+        __syncthreads();
+        if (t < cnt) gx.perm[L.mv_dst[t]] = pv;
     }
 }
 
@@ -752,95 +817,64 @@ __global__ __launch_bounds__(PT) void tslu_final_kernel(int m, int kb, T *__rest
 
 // The update inside an outer panel after a 32-column sub-panel, fused into
 // one launch: with L11 the sub-panel's unit-lower 32 x 32 block at P0 and
-// B12 the 32 x rest block to its right, U12 = L11^{-1} B12 (each workgroup
-// solves all of it for itself: lane c solves column c in registers, the
-// coefficients -- the same for every lane -- come through scalar loads of
-// L11, a read-only view of P0), then the workgroup's 64-row tile of A22
-// (rows 32 + 64 b..) takes A22 -= L21 U12.  U12 goes to `ust` (row stride ldst), not into
-// B12: other workgroups may still be reading B12; the driver copies it into
-// place once the outer panel is done (nothing reads it before).
-// Rows per workgroup: 32 in FP32, so that the kernel's LDS (the tile of L21
-// and U12) fits beside two workgroups of the FP32 lookahead GEMM on a CU
-// (it runs while that GEMM does; with 64 rows it would wait for GEMM tiles
-// to retire).
+// B12 the 32 x rest block to its right, U12 = L11^{-1} B12, then the
+// workgroup's 32-row tile of A22 (rows 32 + 32 b ..) takes A22 -= L21 U12.
+// Thread c owns column c: it solves U12's column c in registers (every
+// workgroup solves all of U12 for itself, so no workgroup waits on
+// another) and then updates column c of its tile.  L11 (transposed, so the
+// coefficients of one step are consecutive) and the tile of L21 sit in LDS
+// and are read as broadcasts.  U12 goes to `ust` (row stride ldst), not
+// into B12: other workgroups may still be reading B12; the driver copies it
+// into place once the outer panel is done (nothing reads it before).
 template <class T>
 constexpr int iu_rows() {
-    return sizeof(T) == 4 ? 32 : 64;
+    return 32;
 }
 constexpr int IU_MAXREST = NB_OUTER - KB;
 template <class T>
 __global__ __launch_bounds__(256) void inner_update_kernel(int rest, int mrows, T *__restrict__ P0, int64_t lda,
-                                                           const T *__restrict__ L11, T *__restrict__ ust,
-                                                           int64_t ldst) {
+                                                           T *__restrict__ ust, int64_t ldst) {
     critical_path_priority();
-    constexpr int kb = KB;
-    constexpr int ROWS = iu_rows<T>(), RG = ROWS / 4;  // rows per workgroup, per group of 64 threads
-    __shared__ __attribute__((aligned(16))) T L2[ROWS][KB];
-    __shared__ T Us[KB][IU_MAXREST + 1];
+    static_assert(IU_MAXREST <= 256, "one thread per column");
+    constexpr int ROWS = iu_rows<T>();
+    __shared__ __attribute__((aligned(16))) T LsT[KB][KB];   // LsT[k][i] = l_ik
+    __shared__ __attribute__((aligned(16))) T L2[ROWS][KB];  // the tile of L21
     const int t = threadIdx.x;
-    const int64_t rt = (int64_t)kb + (int64_t)blockIdx.x * ROWS;  // first tile row, panel-relative
+    const int64_t rt = KB + (int64_t)blockIdx.x * ROWS;  // first tile row, panel-relative
+    for (int e = t; e < KB * KB; e += 256) {
+        const int i = e / KB, k = e % KB;
+        LsT[k][i] = P0[(int64_t)i * lda + k];
+    }
     for (int e = t; e < ROWS * KB; e += 256) {
         const int i = e / KB, k = e % KB;
-        L2[i][k] = (rt + i < kb + mrows && k < kb) ? P0[(rt + i) * lda + k] : T(0);
+        L2[i][k] = rt + i < KB + mrows ? P0[(rt + i) * lda + k] : T(0);
     }
-    T x[KB];
-    const T *B12 = P0 + kb;
+    const bool act = t < rest;
+    T x[KB], pre[ROWS];
 #pragma unroll
-    for (int i = 0; i < KB; ++i) x[i] = (t < rest && i < kb) ? B12[(int64_t)i * lda + t] : T(0);
-    // this thread's A22 elements (column cl + 64 j, rows g*RG + r of the
-    // tile), loaded now so their latency overlaps the solve
-    const int g = t >> 6, cl = t & 63;
-    constexpr int NJ = (IU_MAXREST + 63) / 64;
-    T pre[NJ][RG];
+    for (int i = 0; i < KB; ++i) x[i] = act ? P0[(int64_t)i * lda + KB + t] : T(0);
+    // this thread's column of the A22 tile, loaded now so its latency
+    // overlaps the solve
 #pragma unroll
-    for (int j = 0; j < NJ; ++j)
-#pragma unroll
-        for (int r = 0; r < RG; ++r) {
-            const int c = cl + 64 * j;
-            const int64_t row = rt + g * RG + r;
-            pre[j][r] = (c < rest && row < kb + mrows) ? P0[row * lda + kb + c] : T(0);
-        }
+    for (int r = 0; r < ROWS; ++r) pre[r] = act && rt + r < KB + mrows ? P0[(rt + r) * lda + KB + t] : T(0);
     __syncthreads();
-    // forward substitution with the unit lower L11: x_i -= l_ik x_k, k ascending
+    if (!act) return;
+    // forward substitution with the unit lower L11, a column at a time:
+    // x_i -= l_ik x_k, every x_i taking its terms in ascending k
 #pragma unroll
-    for (int i = 1; i < KB; ++i) {
-        T sacc = x[i];
+    for (int k = 0; k < KB - 1; ++k)
 #pragma unroll
-        for (int k = 0; k < i; ++k) sacc = fma(-L11[(int64_t)i * lda + k], x[k], sacc);
-        x[i] = sacc;
-    }
-    if (t < rest) {
+        for (int i = k + 1; i < KB; ++i) x[i] = fma(-LsT[k][i], x[k], x[i]);
+    if (blockIdx.x == 0)
 #pragma unroll
-        for (int i = 0; i < KB; ++i) Us[i][t] = x[i];
-        if (blockIdx.x == 0)
-            for (int i = 0; i < kb; ++i) ust[(int64_t)i * ldst + t] = x[i];
-    }
-    __syncthreads();
-    // A22 tile -= L21 U12: thread (g, c) takes RG rows of column c
+        for (int i = 0; i < KB; ++i) ust[(int64_t)i * ldst + t] = x[i];
+    // A22 tile -= L21 U12, each product summed in ascending k
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-        const int c = cl + 64 * j;
-        if (c >= rest) break;
-        T acc[RG];
+    for (int r = 0; r < ROWS; ++r) {
+        T acc = T(0);
 #pragma unroll
-        for (int r = 0; r < RG; ++r) acc[r] = T(0);
-#pragma unroll 2
-        for (int k = 0; k < KB; k += 4) {
-            const T u0 = Us[k][c], u1 = Us[k + 1][c], u2 = Us[k + 2][c], u3 = Us[k + 3][c];
-#pragma unroll
-            for (int r = 0; r < RG; ++r) {
-                const T *l = &L2[g * RG + r][k];
-                acc[r] = fma(l[0], u0, acc[r]);
-                acc[r] = fma(l[1], u1, acc[r]);
-                acc[r] = fma(l[2], u2, acc[r]);
-                acc[r] = fma(l[3], u3, acc[r]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RG; ++r) {
-            const int64_t row = rt + g * RG + r;
-            if (row < kb + mrows) P0[row * lda + kb + c] = pre[j][r] - acc[r];
-        }
+        for (int k = 0; k < KB; ++k) acc = fma(L2[r][k], x[k], acc);
+        if (rt + r < KB + mrows) P0[(rt + r) * lda + KB + t] = pre[r] - acc;
     }
 }
 
@@ -1033,8 +1067,8 @@ __global__ void set_i32_kernel(int32_t *p, int32_t v) {
 }
 
 template <class T>
-void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32_t *ipiv, int32_t *srcmap,
-                  int32_t *moves, int32_t *info, PanelSync<T> *sy, unsigned &base, bool tie_last) {
+void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32_t *ipiv, int32_t *info,
+                  PanelSync<T> *sy, unsigned &base, bool tie_last, const PanelGather<T> &gx) {
     int P = (int)cdiv(m, PANEL_ROWS);
     if (P > MAXP) fail_arg("getrf: matrix too tall for the panel kernel");
     TimedRegion tr("getrf_panel", s);
@@ -1048,11 +1082,11 @@ void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32
     // other kernels cost about 11 us (about 10 ms per FP32 factorisation at
     // n = 16384).
     if (tie_last)
-        hipLaunchKernelGGL((getf2_panel_kernel<T, true>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
-                           moves, info, sy, P, base);
+        hipLaunchKernelGGL((getf2_panel_kernel<T, true>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, info,
+                           sy, P, base, gx);
     else
-        hipLaunchKernelGGL((getf2_panel_kernel<T, false>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, srcmap,
-                           moves, info, sy, P, base);
+        hipLaunchKernelGGL((getf2_panel_kernel<T, false>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, info,
+                           sy, P, base, gx);
     LMX_LAUNCH_CHECK();
     base += (unsigned)kb;  // sequence numbers stay monotonic across launches
 }
@@ -1187,24 +1221,30 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             const int sb = std::min(KB, k0 + nbw - s0);
             const int m = N - s0;
             T *P0 = A + (int64_t)s0 * lda + s0;
+            // the row interchanges reach this outer panel's other columns
+            // at once, the rest later (move list kept)
+            int32_t *perm_s = d_perm ? d_perm + s0 : nullptr;
+            int32_t *pairs_s = pairs + (int64_t)np * PAIR_STRIDE;
+            bool tslu_done = false;
             if constexpr (sizeof(T) == 4) {
-                if (tournament)
+                if (tournament) {
                     launch_tslu<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, ts);
-                else
-                    launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base, tie_last);
-            } else {
-                launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, srcmap, moves, d_info, sy, base, tie_last);
+                    gather_rows<T>(s, A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, moves, srcmap, perm_s,
+                                   pairs_s, s0);
+                    tslu_done = true;
+                }
             }
-            // this outer panel's columns now, the rest later (move list kept)
-            gather_rows<T>(s, A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, moves, srcmap,
-                           d_perm ? d_perm + s0 : nullptr, pairs + (int64_t)np * PAIR_STRIDE, s0);
+            if (!tslu_done) {
+                const PanelGather<T> gx{A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, perm_s, pairs_s};
+                launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, d_info, sy, base, tie_last, gx);
+            }
             const int rest = k0 + nbw - (s0 + sb);
             if (rest > 0) {
                 // U12 into the stash at (s0 - k0, s0 - k0 + sb) of the outer block
                 T *ust = stash + (int64_t)(s0 - k0) * NB_OUTER + (s0 - k0) + sb;
                 if (sb != KB) fail_arg("getrf: inner update needs a full sub-panel");  // rest > 0 implies sb = KB
                 hipLaunchKernelGGL(inner_update_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(m - sb, iu_rows<T>()))),
-                                   dim3(256), 0, s, rest, m - sb, P0, lda, (const T *)P0, ust, (int64_t)NB_OUTER);
+                                   dim3(256), 0, s, rest, m - sb, P0, lda, ust, (int64_t)NB_OUTER);
                 LMX_LAUNCH_CHECK();
             }
         }
