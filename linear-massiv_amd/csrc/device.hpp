@@ -48,6 +48,14 @@ template <int CTRL, int RM>
 __device__ inline double dpp_f(double v) {
     return __hiloint2double(dpp_i<CTRL, RM>(__double2hiint(v)), dpp_i<CTRL, RM>(__double2loint(v)));
 }
+// Lane k's value to every lane (k a compile-time or wave-uniform lane).
+__device__ inline float bcast_lane(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ inline double bcast_lane(double v, int k) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), k),
+                            __builtin_amdgcn_readlane(__double2loint(v), k));
+}
 __device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 __device__ inline double lane63(double v) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),

@@ -101,7 +101,17 @@ struct PanelRows {
     int tag[RPT];
 };
 
-// LDS of the panel kernel (double-buffered by column parity).
+// Columns a pass of the panel kernel's closing row gather: the U12 block
+// of a pass (KB x GCOLS) shares LDS with the poll payloads.
+template <class T>
+constexpr int gather_cols() {
+    return sizeof(T) == 4 ? 128 : 64;
+}
+
+// LDS of the panel kernel.  What is written before a column's first
+// barrier (wave slots, row j) is double-buffered by column parity; the poll
+// payloads are written after it and read before the next column's, so one
+// buffer serves.
 template <class T>
 struct PanelLds {
     alignas(16) T slot_data[2][PW][KB];  // each wave's best row
@@ -111,7 +121,11 @@ struct PanelLds {
     T drow[2][KB];                 // row j (single workgroup)
     int dtag[2];
     int pivs[KB];
-    alignas(8) unsigned pay[2][PanelRec<T>::TOTAL];  // polled record payloads (several workgroups)
+    T l11[KB][KB];                 // the panel's unit-lower L11: row j recorded at column j
+    union {
+        alignas(8) unsigned pay[PanelRec<T>::TOTAL];  // polled record payloads (several workgroups)
+        alignas(16) T b12[KB][gather_cols<T>()];      // closing gather: B12, then U12
+    };
     int mv_dst[2 * KB];  // the rows that move (panel-relative) ...
     int mv_src[2 * KB];  // ... and the original row each one receives
     int mv_cnt;
@@ -273,7 +287,7 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
                     wv[i] = xload(wp);
                 }
                 const int dst = e < P * R::W ? e : MAXP * R::W + (e - P * R::W);
-                L.pay[par][dst] = (unsigned)wv[i];
+                L.pay[dst] = (unsigned)wv[i];
             }
         }
         __syncthreads();
@@ -281,7 +295,7 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
         //    ascending rows, so the tie rule over rows is the one over
         //    workgroups.  Every wave reduces the P records itself (lane x
         //    reads record x).  Record values are -1 or a magnitude, never NaN.
-        const unsigned *pay = L.pay[par];
+        const unsigned *pay = L.pay;
         const T xv = lane < P ? *reinterpret_cast<const T *>(pay + lane * R::W) : T(-1);
         const T gb = wave_max(xv);
         const int gw = tie_index<TL>(__ballot(lane < P && xv == gb && gb >= T(0)));
@@ -297,6 +311,9 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
             ptag = (int)pay[gw * R::W + R::WPV + 1];
         }
     }
+    // row j of L11: the pivot row's multipliers of columns k < j (final
+    // since column k; register (k - j0) mod KB in the rotated frame)
+    if (t < j) L.l11[j][t] = prow[(t - j0) & (KB - 1)];
     // 6. interchange rows j and p
     if (p != j) {
         if (row == p) {
@@ -354,6 +371,8 @@ struct RowVec<double> {
 // (rows relative to the panel's first row): columns [c0, c1) and [c2, c3)
 // of G, and the permutation vector perm when not null; pairs_out receives
 // the move list (rows absolute) for the columns left for later.
+//   With u12 set, rows 0 .. KB - 1 of columns [c2, c3) (B12 of a full
+// panel) receive U12 = L11^{-1} B12 instead of the gathered B12.
 template <class T>
 struct PanelGather {
     T *G;
@@ -361,8 +380,11 @@ struct PanelGather {
     int c0, c1, c2, c3;
     int32_t *perm;
     int32_t *pairs_out;
+    int u12;
 };
 
+// At most 128 VGPRs (four waves a SIMD), so that GEMM waves of the other
+// stream still find registers on the panel's CUs.
 template <class T, bool TL>
 __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
                                                           int k0, int32_t *__restrict__ ipiv,
@@ -452,8 +474,11 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
     //  b. the original row each of them receives: lane e holds the tag of
     //     the row at position mv_dst[e] and the kb interchanges are
     //     replayed on those tags;
-    //  c. the gather itself, 128 columns a pass, each column read by one
-    //     workgroup before it is written.
+    //  c. the gather itself, gather_cols() columns a pass, each column read
+    //     by one workgroup before it is written; for B12's columns one
+    //     thread a column then solves U12 = L11^{-1} B12 (unit lower,
+    //     x_i -= l_ik x_k with k ascending for every i) between the reads
+    //     and the writes.
     const int nd = min(kb, m);
     if (t < 64) {
         const int pj = t < kb ? L.pivs[t] : -1;
@@ -492,21 +517,54 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
         gx.pairs_out[1 + 2 * KB + t] = k0 + L.mv_src[t];
     }
     const int n1 = gx.c1 - gx.c0, ncols = n1 + (gx.c3 - gx.c2);
-    constexpr int GCOLS = 128, GPER = 2 * KB / (PT / GCOLS);  // columns a pass; moves per thread
-    for (int cb = w * GCOLS; cb < ncols; cb += P * GCOLS) {
-        const int cc = cb + (t % GCOLS), grp = t / GCOLS;
-        const int col = cc < n1 ? gx.c0 + cc : (cc < ncols ? gx.c2 + (cc - n1) : -1);
+    constexpr int GCOLS = gather_cols<T>(), GPER = 2 * KB / (PT / GCOLS);  // columns a pass; moves per thread
+    const bool u12 = gx.u12 != 0 && kb == KB;
+    const int lane = t & 63, wave = t >> 6;
+    // for the U12 solve: lane i holds row i of L11 (columns k < i)
+    T lrow[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) lrow[k] = (u12 && lane < KB && k < lane) ? L.l11[lane][k] : T(0);
+    // workgroup w takes a contiguous chunk of the columns
+    const int chunk = (ncols + P - 1) / P;
+    const int cbeg = w * chunk, cend = min(ncols, cbeg + chunk);
+    for (int cb = cbeg; cb < cend; cb += GCOLS) {
+        const int tc = t % GCOLS, cc = cb + tc, grp = t / GCOLS;
+        const int col = cc >= cend ? -1 : (cc < n1 ? gx.c0 + cc : gx.c2 + (cc - n1));
+        const bool ucol = u12 && cc >= n1 && col >= 0;  // a column of B12
+        // a wave's threads share grp, so the rows are wave-uniform
         T val[GPER];
 #pragma unroll
         for (int i = 0; i < GPER; ++i) {
             const int e = grp * GPER + i;
-            val[i] = (col >= 0 && e < cnt) ? gx.G[(int64_t)L.mv_src[e] * gx.ldg + col] : T(0);
+            const T *src = gx.G + (int64_t)__builtin_amdgcn_readfirstlane(L.mv_src[e]) * gx.ldg;
+            val[i] = (col >= 0 && e < cnt) ? src[col] : T(0);
+            // positions 0 .. KB - 1 are the first KB entries of the list
+            if (ucol && e < KB) L.b12[e][tc] = val[i];
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < GPER; ++i) {
             const int e = grp * GPER + i;
-            if (col >= 0 && e < cnt) gx.G[(int64_t)L.mv_dst[e] * gx.ldg + col] = val[i];
+            T *dst = gx.G + (int64_t)__builtin_amdgcn_readfirstlane(L.mv_dst[e]) * gx.ldg;
+            if (col >= 0 && e < cnt && !(ucol && e < KB)) dst[col] = val[i];
+        }
+        if (u12) {
+            // U12 = L11^{-1} B12, a column per wave: lane i holds x_i, and
+            // step k broadcasts x_k (by v_readlane) to x_i -= l_ik x_k for
+            // the lanes i > k -- every x_i takes its terms in ascending k.
+            // Rows 0 .. KB - 1 are positions 0 .. KB - 1.
+            for (int q = wave; q < GCOLS; q += PW) {
+                const int cq = cb + q;
+                if (cq >= cend || cq < n1) continue;
+                T x = lane < KB ? L.b12[lane][q] : T(0);
+#pragma unroll
+                for (int k = 0; k < KB - 1; ++k) {
+                    const T xk = bcast_lane(x, k);
+                    x = lane > k ? fma(-lrow[k], xk, x) : x;
+                }
+                if (lane < KB) (gx.G + (int64_t)lane * gx.ldg)[gx.c2 + (cq - n1)] = x;
+            }
+            __syncthreads();  // the next pass rewrites b12
         }
     }
     if (w == 0 && gx.perm != nullptr) {
@@ -1235,20 +1293,27 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
                 }
             }
             if (!tslu_done) {
-                const PanelGather<T> gx{A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, perm_s, pairs_s};
+                const PanelGather<T> gx{A + (int64_t)s0 * lda, lda, k0, s0, s0 + sb, k0 + nbw, perm_s, pairs_s, 1};
                 launch_panel<T>(s, m, sb, P0, lda, s0, d_ipiv, d_info, sy, base, tie_last, gx);
             }
             const int rest = k0 + nbw - (s0 + sb);
             if (rest > 0) {
-                // U12 into the stash at (s0 - k0, s0 - k0 + sb) of the outer block
-                T *ust = stash + (int64_t)(s0 - k0) * NB_OUTER + (s0 - k0) + sb;
                 if (sb != KB) fail_arg("getrf: inner update needs a full sub-panel");  // rest > 0 implies sb = KB
-                hipLaunchKernelGGL(inner_update_kernel<T>, dim3((unsigned)std::max<int64_t>(1, cdiv(m - sb, iu_rows<T>()))),
-                                   dim3(256), 0, s, rest, m - sb, P0, lda, ust, (int64_t)NB_OUTER);
-                LMX_LAUNCH_CHECK();
+                if (tslu_done) {
+                    // U12 into the stash at (s0 - k0, s0 - k0 + sb) of the outer block
+                    T *ust = stash + (int64_t)(s0 - k0) * NB_OUTER + (s0 - k0) + sb;
+                    hipLaunchKernelGGL(inner_update_kernel<T>,
+                                       dim3((unsigned)std::max<int64_t>(1, cdiv(m - sb, iu_rows<T>()))), dim3(256), 0,
+                                       s, rest, m - sb, P0, lda, ust, (int64_t)NB_OUTER);
+                    LMX_LAUNCH_CHECK();
+                } else if (m > sb) {
+                    // the panel kernel left U12 in place: A22 -= L21 U12
+                    gemm(s, false, false, m - sb, rest, sb, T(-1), P0 + (int64_t)sb * lda, lda, P0 + sb, lda, T(1),
+                            P0 + (int64_t)sb * lda + sb, lda);
+                }
             }
         }
-        if (nbw > KB) {
+        if (nbw > KB && tournament) {
             hipLaunchKernelGGL(inner_flush_kernel<T>, dim3((unsigned)cdiv((int64_t)nbw * nbw, 256)), dim3(256), 0, s,
                                nbw, stash, (int64_t)NB_OUTER, A + (int64_t)k0 * lda + k0, lda);
             LMX_LAUNCH_CHECK();
