@@ -2,30 +2,31 @@
 // LU solve, for row-major matrices.
 //
 // Panel kernel (getf2_panel_kernel): the m x kb panel (kb <= 32) is spread
-// over P concurrent workgroups of PT = 512 threads, each owning PT*RPT
-// consecutive rows held in registers.  Per column j:
+// over P concurrent workgroups of 512 rows each (FP32: 4 waves, two rows a
+// thread; FP64: 8 waves, one row a thread), rows held in registers.  Per
+// column j:
 //   1. each workgroup finds its candidate max |a_ij| over rows i >= j (ties:
 //      lowest row, NaN never chosen -- the rule of rawPivotSearch,
 //      src/.../Internal/Kernel.hs:1541-1556; or, for the generic lu, the
 //      highest row among equal magnitudes -- maximumBy keeps the later of
 //      two equal elements, src/.../Solve/LU.hs:128-135): DPP wave
-//      reductions, then one LDS exchange between the waves;
+//      reductions and ballots, then one LDS exchange between the waves;
 //   2. it publishes the candidate (value, row, full row data) in slot j%2,
-//      and workgroup 0 publishes row j, then raises its sequence number;
-//   3. every workgroup waits for all P sequence numbers, reduces the P
-//      candidates to the pivot p, swaps rows j and p, and all rows i > j
-//      get l = a_ij / a_pj (a division, as the reference) and the rank-1
-//      update of the remaining panel columns.
-// The current column always sits in register 0: after each column the
-// row's registers rotate left by one, so the column loop needs no dynamic
-// register indexing.  Each row carries its panel-relative origin (a tag);
-// at the end the tags give the row-move map of the panel, and the rest of
-// the matrix (and the permutation vector) is permuted with one parallel
-// gather instead of a sequential swap loop.
+//      and workgroup 0 publishes row j, each word tagged with the column;
+//   3. every workgroup polls all P records, reduces them to the pivot p,
+//      swaps rows j and p, and all rows i > j get l = a_ij / a_pj (a
+//      division, as the reference) and the rank-1 update of the remaining
+//      panel columns.
+// The current column always sits in register 0..3: after every 4 columns
+// the row's registers rotate left by 4, so the column loop needs no dynamic
+// register indexing.  At the end every workgroup derives the panel's
+// row-move list from the pivots and applies it to the other columns of the
+// outer panel (and the permutation vector) with one parallel gather, and
+// solves U12 = L11^{-1} B12 for the columns to the right of the panel.
 //
 // Driver (getrf_impl): outer panels of NB = 256 columns, each factored as
-// eight 32-column sub-panels (panel kernel, gather, TRSM + GEMM update
-// inside the outer panel), then U12 = L11^{-1} A12 (TRSM through the
+// eight 32-column sub-panels (panel kernel, then A22 -= L21 U12 inside the
+// outer panel on the MFMA GEMM), then U12 = L11^{-1} A12 (TRSM through the
 // inverse of L11) and the trailing update A22 -= L21 U12 on the MFMA GEMM.
 #include <climits>
 #include <vector>
@@ -54,12 +55,12 @@ constexpr long SPIN_LIMIT = 1L << 22;
 // word until its tag matches, so the poll is the load and no flag is needed.
 // Tags grow over the whole factorisation (the records are zeroed once per
 // getrf) and records are double-buffered by column parity.
-//   workgroup record: value (WPV words), row, tag, pad to even, row data
-//   row-j record:     tag, pad, row data
+//   workgroup record: value (WPV words), row, pad, row data
+//   row-j record:     pad, pad, row data
 template <class T>
 struct PanelRec {
     static constexpr int WPV = sizeof(T) / 4;
-    static constexpr int DATA = WPV + 2 + (WPV == 2 ? 0 : 0);  // data offset (even for doubles)
+    static constexpr int DATA = WPV + 2;  // data offset (even for doubles)
     static constexpr int W = DATA + KB * WPV;                  // words per workgroup record
     static constexpr int DDATA = 2;
     static constexpr int DW = DDATA + KB * WPV;                // words of the row-j record
@@ -67,8 +68,8 @@ struct PanelRec {
 };
 template <class T>
 struct PanelSync {
-    unsigned long long rec[2][MAXP * PanelRec<T>::W];
-    unsigned long long drow[2][PanelRec<T>::DW];
+    // the P workgroup records, then the row-j record (at P * W)
+    unsigned long long rec[2][PanelRec<T>::TOTAL];
     unsigned error;
     unsigned pad[63];
 };
@@ -91,21 +92,34 @@ __device__ inline unsigned word_of(T v, int h) {
 
 constexpr int PG = 4;  // columns per register rotation
 
-// Per-thread state of the panel kernel: RPT rows of the panel in registers,
-// in the rotated frame (register c holds column j0 + c of the current
-// group), each with its panel-relative origin (tag).
+// Shape of the partial-pivoting panel kernel's workgroup (PANEL_ROWS rows):
+// 8 waves of one row a thread.  A wave's rows are consecutive: row block q
+// of a wave is rows 64 q .. 64 q + 63 of its 64 RPT.  (Two rows a thread in
+// 4 waves, and a one-row panel held to 85 registers, both let a panel
+// workgroup share its CU with two workgroups of the FP32 lookahead GEMM;
+// both measured slower beside that GEMM.)
 template <class T>
+struct PanelShape {
+    static constexpr int RPT = 1;               // rows per thread
+    static constexpr int PT = PANEL_ROWS / RPT;  // threads
+    static constexpr int PW = PT / 64;           // waves
+    static constexpr int MIN_WAVES_PER_SIMD = 2;
+};
+
+// Per-thread state of the panel kernel: R rows of the panel in registers,
+// in the rotated frame (register c holds column j0 + c of the current
+// group).
+template <class T, int R>
 struct PanelRows {
-    T v[RPT][KB];
-    int row[RPT];
-    int tag[RPT];
+    T v[R][KB];
+    int row[R];
 };
 
 // Columns a pass of the panel kernel's closing row gather: the U12 block
 // of a pass (KB x GCOLS) shares LDS with the poll payloads.
 template <class T>
 constexpr int gather_cols() {
-    return sizeof(T) == 4 ? 128 : 64;
+    return 64;
 }
 
 // LDS of the panel kernel.  What is written before a column's first
@@ -114,12 +128,11 @@ constexpr int gather_cols() {
 // buffer serves.
 template <class T>
 struct PanelLds {
-    alignas(16) T slot_data[2][PW][KB];  // each wave's best row
-    T slot_val[2][PW];
-    int slot_row[2][PW];
-    int slot_tag[2][PW];
+    static constexpr int NW = PanelShape<T>::PW;
+    alignas(16) T slot_data[2][NW][KB];  // each wave's best row
+    T slot_val[2][NW];
+    int slot_row[2][NW];
     T drow[2][KB];                 // row j (single workgroup)
-    int dtag[2];
     int pivs[KB];
     T l11[KB][KB];                 // the panel's unit-lower L11: row j recorded at column j
     union {
@@ -178,74 +191,96 @@ __device__ __forceinline__ int tie_index(unsigned long long mask) {
 // One column (j = j0 + U) of the panel: pivot search, interchange, scaling
 // and rank-1 update of the rest of the panel.
 template <class T, int U, bool TL>
-__device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, int j0, int kb, int m, int P,
-                                             PanelSync<T> *__restrict__ sy, unsigned base, int k0,
+__device__ __forceinline__ void panel_column(PanelRows<T, PanelShape<T>::RPT> &st, PanelLds<T> &L, int j0, int kb,
+                                             int m, int P, PanelSync<T> *__restrict__ sy, unsigned base, int k0,
                                              int32_t *__restrict__ ipiv, int &local_info, bool &failed) {
-    static_assert(RPT == 1, "one panel row per thread");
+    typedef PanelShape<T> S;
+    constexpr int NR = S::RPT;
     typedef PanelRec<T> R;
     typedef TieRule<TL> Tie;
     const int j = j0 + U;
     if (j >= kb) return;
     const int w = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int par = j & 1;
-    const int row = st.row[0];
-    const bool live = row >= j && row < m;
     // 1. this wave's candidate: max |a_ij| over its rows i >= j (fmax skips
-    //    NaN), then the tie rule's lane among the rows attaining it (a
-    //    wave's lanes hold consecutive rows, so lane order is row order);
-    //    that lane writes its row to the wave slot
-    const T a = live ? fabs(st.v[0][U]) : T(-1);
-    const T wmax = wave_max(a);
-    const int wl = tie_index<TL>(__ballot(a == wmax && a >= T(0)));
+    //    NaN), then the tie rule's row among those attaining it (row block
+    //    q of a wave precedes block q + 1, lanes ascend within a block);
+    //    the owning lane writes that row to the wave slot
+    bool live[NR];
+    T a[NR];
+    T tmax = T(-1);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        live[q] = st.row[q] >= j && st.row[q] < m;
+        a[q] = live[q] ? fabs(st.v[q][U]) : T(-1);
+        tmax = fmax(tmax, a[q]);
+    }
+    const T wmax = wave_max(tmax);
+    int wl = -1, wq = 0;
+#pragma unroll
+    for (int qq = 0; qq < NR; ++qq) {
+        const int q = TL ? NR - 1 - qq : qq;
+        const unsigned long long mk = __ballot(a[q] == wmax && a[q] >= T(0));
+        if (wl < 0 && mk != 0) {
+            wl = tie_index<TL>(mk);
+            wq = q;
+        }
+    }
     if (lane == wl) {
 #pragma unroll
-        for (int c = 0; c < KB; ++c) L.slot_data[par][wave][c] = st.v[0][c];
-        L.slot_tag[par][wave] = st.tag[0];
-    }
-    if (row == j) {
+        for (int c = 0; c < KB; ++c) {
+            T x = st.v[0][c];
 #pragma unroll
-        for (int c = 0; c < KB; ++c) L.drow[par][c] = st.v[0][c];
-        L.dtag[par] = st.tag[0];
+            for (int q = 1; q < NR; ++q) x = wq == q ? st.v[q][c] : x;
+            L.slot_data[par][wave][c] = x;
+        }
     }
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        if (st.row[q] == j) {
+#pragma unroll
+            for (int c = 0; c < KB; ++c) L.drow[par][c] = st.v[q][c];
+        }
     if (lane == 0) {
         // a wave with no candidate offers -1, so no slot holds a NaN
         L.slot_val[par][wave] = wl < 0 ? T(-1) : wmax;
-        L.slot_row[par][wave] = wl < 0 ? Tie::NONE : w * PANEL_ROWS + wave * 64 + wl;
+        L.slot_row[par][wave] = wl < 0 ? Tie::NONE : w * PANEL_ROWS + wave * 64 * NR + wq * 64 + wl;
     }
     __syncthreads();
     // 2. this workgroup's candidate: lane x reads slot x; waves hold
     //    ascending rows, so the tie rule over rows is the one over waves
-    const T sv = lane < PW ? L.slot_val[par][lane] : T(-1);
+    const T sv = lane < S::PW ? L.slot_val[par][lane] : T(-1);
     const T best = wave_max(sv);
-    const int bw = tie_index<TL>(__ballot(lane < PW && sv == best && best >= T(0)));
+    const int bw = tie_index<TL>(__ballot(lane < S::PW && sv == best && best >= T(0)));
     const int brow = bw < 0 ? Tie::NONE : L.slot_row[par][bw];
-    int p, ptag, dtag;
+    int p;
     const T *prow;
     const T *drow;
     if (P == 1) {
         drow = L.drow[par];
-        dtag = L.dtag[par];
         // a NaN at row j keeps row j, as rawPivotSearch does
         if (bw < 0 || isnan(drow[U])) {
             p = j;
             prow = drow;
-            ptag = dtag;
         } else {
             p = brow;
             prow = L.slot_data[par][bw];
-            ptag = L.slot_tag[par][bw];
         }
     } else {
         const unsigned tg = base + (unsigned)j + 1u;
+        // The records' base, opaque to the compiler: every address below is
+        // formed here, in the column, rather than hoisted out of the column
+        // loop into registers held across all of it.
+        unsigned long long *rbase = sy->rec[par];
+        asm volatile("" : "+v"(rbase));
         // 3. publish this workgroup's record (and row j from workgroup 0):
         //    lane e of wave 0 writes word e
         if (wave == 0) {
-            unsigned long long *rec = sy->rec[par] + w * R::W;
+            unsigned long long *rec = rbase + w * R::W;
             const unsigned *data = reinterpret_cast<const unsigned *>(L.slot_data[par][bw < 0 ? 0 : bw]);
-            const unsigned btag = bw < 0 ? 0u : (unsigned)L.slot_tag[par][bw];
             for (int e = lane; e < R::W; e += 64) {
                 const unsigned dv = data[max(e - R::DATA, 0)];
-                const unsigned hv = e < R::WPV ? word_of(best, e) : (e == R::WPV ? (unsigned)brow : btag);
+                const unsigned hv = e < R::WPV ? word_of(best, e) : (e == R::WPV ? (unsigned)brow : 0u);
                 const unsigned pl = e >= R::DATA ? (bw < 0 ? 0u : dv) : hv;
                 xstore(rec + e, tagword(pl, tg));
             }
@@ -253,29 +288,27 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
                 const unsigned *dd = reinterpret_cast<const unsigned *>(L.drow[par]);
                 for (int e = lane; e < R::DW; e += 64) {
                     const unsigned dv = dd[max(e - R::DDATA, 0)];
-                    const unsigned pl = e >= R::DDATA ? dv : (e == 0 ? (unsigned)L.dtag[par] : 0u);
-                    xstore(sy->drow[par] + e, tagword(pl, tg));
+                    xstore(rbase + P * R::W + e, tagword(e >= R::DDATA ? dv : 0u, tg));
                 }
             }
         }
         // 4. poll every record: all loads in flight first, then the
         //    stragglers one by one
-        constexpr int NPOLL = (R::TOTAL + PT - 1) / PT;
+        constexpr int NPOLL = (R::TOTAL + S::PT - 1) / S::PT;
         const int total = P * R::W + R::DW;
         unsigned long long wv[NPOLL];
 #pragma unroll
         for (int i = 0; i < NPOLL; ++i) {
-            const int e = t + i * PT;
+            const int e = t + i * S::PT;
             if (e < total) {
-                const unsigned long long *wp = e < P * R::W ? sy->rec[par] + e : sy->drow[par] + (e - P * R::W);
-                wv[i] = xload(wp);
+                wv[i] = xload(rbase + e);
             }
         }
 #pragma unroll
         for (int i = 0; i < NPOLL; ++i) {
-            const int e = t + i * PT;
+            const int e = t + i * S::PT;
             if (e < total) {
-                const unsigned long long *wp = e < P * R::W ? sy->rec[par] + e : sy->drow[par] + (e - P * R::W);
+                const unsigned long long *wp = rbase + e;
                 long spins = 0;
                 while ((unsigned)(wv[i] >> 32) != tg && !failed) {
                     __builtin_amdgcn_s_sleep(1);
@@ -286,8 +319,7 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
                     }
                     wv[i] = xload(wp);
                 }
-                const int dst = e < P * R::W ? e : MAXP * R::W + (e - P * R::W);
-                L.pay[dst] = (unsigned)wv[i];
+                L.pay[e] = (unsigned)wv[i];
             }
         }
         __syncthreads();
@@ -299,16 +331,13 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
         const T xv = lane < P ? *reinterpret_cast<const T *>(pay + lane * R::W) : T(-1);
         const T gb = wave_max(xv);
         const int gw = tie_index<TL>(__ballot(lane < P && xv == gb && gb >= T(0)));
-        drow = reinterpret_cast<const T *>(pay + MAXP * R::W + R::DDATA);
-        dtag = (int)pay[MAXP * R::W];
+        drow = reinterpret_cast<const T *>(pay + P * R::W + R::DDATA);
         if (gw < 0 || isnan(drow[U])) {
             p = j;
             prow = drow;
-            ptag = dtag;
         } else {
             p = (int)pay[gw * R::W + R::WPV];
             prow = reinterpret_cast<const T *>(pay + gw * R::W + R::DATA);
-            ptag = (int)pay[gw * R::W + R::WPV + 1];
         }
     }
     // row j of L11: the pivot row's multipliers of columns k < j (final
@@ -316,14 +345,15 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     if (t < j) L.l11[j][t] = prow[(t - j0) & (KB - 1)];
     // 6. interchange rows j and p
     if (p != j) {
-        if (row == p) {
 #pragma unroll
-            for (int c = 0; c < KB; ++c) st.v[0][c] = drow[c];
-            st.tag[0] = dtag;
-        } else if (row == j) {
+        for (int q = 0; q < NR; ++q) {
+            if (st.row[q] == p) {
 #pragma unroll
-            for (int c = 0; c < KB; ++c) st.v[0][c] = prow[c];
-            st.tag[0] = ptag;
+                for (int c = 0; c < KB; ++c) st.v[q][c] = drow[c];
+            } else if (st.row[q] == j) {
+#pragma unroll
+                for (int c = 0; c < KB; ++c) st.v[q][c] = prow[c];
+            }
         }
     }
     if (t == 0) L.pivs[j] = p;
@@ -339,17 +369,19 @@ __device__ __forceinline__ void panel_column(PanelRows<T> &st, PanelLds<T> &L, i
     for (int c = U; c < KB; ++c) pr[c] = prow[c];
     const T piv = pr[U];
     if (piv != T(0)) {
-        if (live && row > j) {
-            const T l = st.v[0][U] / piv;
-            st.v[0][U] = l;
 #pragma unroll
-            for (int b = (U + 1) / PG * PG; b < KB; b += PG)
-                if (b < KB - j0) {
+        for (int q = 0; q < NR; ++q)
+            if (live[q] && st.row[q] > j) {
+                const T l = st.v[q][U] / piv;
+                st.v[q][U] = l;
 #pragma unroll
-                    for (int e = 0; e < PG; ++e)
-                        if (b + e > U) st.v[0][b + e] = fma(-l, pr[b + e], st.v[0][b + e]);
-                }
-        }
+                for (int b = (U + 1) / PG * PG; b < KB; b += PG)
+                    if (b < KB - j0) {
+#pragma unroll
+                        for (int e = 0; e < PG; ++e)
+                            if (b + e > U) st.v[q][b + e] = fma(-l, pr[b + e], st.v[q][b + e]);
+                    }
+            }
     } else if (local_info == 0) {
         local_info = k0 + j + 1;
     }
@@ -386,24 +418,25 @@ struct PanelGather {
 // At most 128 VGPRs (four waves a SIMD), so that GEMM waves of the other
 // stream still find registers on the panel's CUs.
 template <class T, bool TL>
-__global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
+__global__ __launch_bounds__(PanelShape<T>::PT)
+__attribute__((amdgpu_waves_per_eu(PanelShape<T>::MIN_WAVES_PER_SIMD))) void getf2_panel_kernel(int m, int kb, T *__restrict__ A, int64_t lda,
                                                           int k0, int32_t *__restrict__ ipiv,
                                                           int32_t *__restrict__ info,
                                                           PanelSync<T> *__restrict__ sy, int P,
                                                           unsigned base, PanelGather<T> gx) {
     critical_path_priority();
+    typedef PanelShape<T> S;
     const int w = blockIdx.x, t = threadIdx.x;
     __shared__ PanelLds<T> L;
-    PanelRows<T> st;
+    PanelRows<T, S::RPT> st;
     // A full-width panel whose rows start on 16-byte boundaries moves each
     // row in 16-byte vectors (a quarter of the address work of 4-byte ones)
     typedef typename RowVec<T>::type V;
     constexpr int VE = 16 / sizeof(T);
     const bool vec = kb == KB && (reinterpret_cast<uintptr_t>(A) & 15) == 0 && ((lda * (int64_t)sizeof(T)) & 15) == 0;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        st.row[q] = w * PANEL_ROWS + q * PT + t;
-        st.tag[q] = st.row[q];
+    for (int q = 0; q < S::RPT; ++q) {
+        st.row[q] = w * PANEL_ROWS + (t >> 6) * 64 * S::RPT + q * 64 + (t & 63);
         if (vec) {
             const T *src = A + (int64_t)st.row[q] * lda;
 #pragma unroll
@@ -429,7 +462,7 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
         panel_column<T, 3, TL>(st, L, j0, kb, m, P, sy, base, k0, ipiv, local_info, failed);
         // rotate: the next group's first column moves to register 0
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
+        for (int q = 0; q < S::RPT; ++q) {
             T keep[PG];
 #pragma unroll
             for (int c = 0; c < PG; ++c) keep[c] = st.v[q][c];
@@ -446,7 +479,7 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
     // (column s itself for a full-width panel: KB / PG rotations).
     const int shift = (PG * groups) % KB;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q)
+    for (int q = 0; q < S::RPT; ++q)
         if (st.row[q] < m) {
             T *dst = A + (int64_t)st.row[q] * lda;
             if (vec) {
@@ -517,7 +550,7 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
         gx.pairs_out[1 + 2 * KB + t] = k0 + L.mv_src[t];
     }
     const int n1 = gx.c1 - gx.c0, ncols = n1 + (gx.c3 - gx.c2);
-    constexpr int GCOLS = gather_cols<T>(), GPER = 2 * KB / (PT / GCOLS);  // columns a pass; moves per thread
+    constexpr int GCOLS = gather_cols<T>(), GPER = 2 * KB / (S::PT / GCOLS);  // columns a pass; moves per thread
     const bool u12 = gx.u12 != 0 && kb == KB;
     const int lane = t & 63, wave = t >> 6;
     // for the U12 solve: lane i holds row i of L11 (columns k < i)
@@ -553,7 +586,7 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(int m, int kb, T *__res
             // step k broadcasts x_k (by v_readlane) to x_i -= l_ik x_k for
             // the lanes i > k -- every x_i takes its terms in ascending k.
             // Rows 0 .. KB - 1 are positions 0 .. KB - 1.
-            for (int q = wave; q < GCOLS; q += PW) {
+            for (int q = wave; q < GCOLS; q += S::PW) {
                 const int cq = cb + q;
                 if (cq >= cend || cq < n1) continue;
                 T x = lane < KB ? L.b12[lane][q] : T(0);
@@ -1140,10 +1173,10 @@ void launch_panel(hipStream_t s, int m, int kb, T *A, int64_t lda, int k0, int32
     // other kernels cost about 11 us (about 10 ms per FP32 factorisation at
     // n = 16384).
     if (tie_last)
-        hipLaunchKernelGGL((getf2_panel_kernel<T, true>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, info,
+        hipLaunchKernelGGL((getf2_panel_kernel<T, true>), dim3(P), dim3(PanelShape<T>::PT), 0, s, m, kb, A, lda, k0, ipiv, info,
                            sy, P, base, gx);
     else
-        hipLaunchKernelGGL((getf2_panel_kernel<T, false>), dim3(P), dim3(PT), 0, s, m, kb, A, lda, k0, ipiv, info,
+        hipLaunchKernelGGL((getf2_panel_kernel<T, false>), dim3(P), dim3(PanelShape<T>::PT), 0, s, m, kb, A, lda, k0, ipiv, info,
                            sy, P, base, gx);
     LMX_LAUNCH_CHECK();
     base += (unsigned)kb;  // sequence numbers stay monotonic across launches
