@@ -36,12 +36,13 @@ PEAK = {"gemm": 78.6, "sgemm": 157.3, "syrk": 78.6, "ssyrk": 157.3, "bgemm": 250
 KERNEL_FAMILIES = ["gemm", "sgemm", "gemm64", "sgemm64", "syrk", "ssyrk", "bgemm", "bgemm3", "getrf_panel", "trsm",
                    "laswp", "potrf_diag"]
 # HBM bytes per launch of a workload's dominant kernel, from the rocprofv3
-# FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (workload, n) -> file
-TRAFFIC_PROFILES = {("lu_mixed", 16384): "sgemm_lu_mixed_n16384_traffic.md"}
+# FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
+# (workload, n, kernel family) -> file
+TRAFFIC_PROFILES = {("lu_mixed", 16384, "sgemm"): "sgemm_lu_mixed_n16384_traffic.md"}
 
 
-def profiled_traffic(workload, n):
-    name = TRAFFIC_PROFILES.get((workload, n))
+def profiled_traffic(workload, n, family):
+    name = TRAFFIC_PROFILES.get((workload, n, family))
     if not name:
         return None
     try:
@@ -603,12 +604,12 @@ def main():
                     "kernel": {"gemm": "gemm_nn_kernel<double, 128, 64>" if wl == "gemm_fp64"
                                else "gemm_nn_kernel<double, 128, 128>",
                                "sgemm": "gemm_mfma_kernel<float, 128, 128, 16>",
-                               "syrk": "gemm_mfma_kernel<double> (lower tiles)",
-                               "ssyrk": "gemm_mfma_kernel<float> (lower tiles)",
+                               "syrk": "gemm_nn_kernel<double, B^T> (lower tiles)",
+                               "ssyrk": "gemm_nn_kernel<float, B^T> (lower tiles)",
                                "bgemm": "gemm_bf16_kernel<bf16, 128, 128>",
                                "bgemm3": "gemm_bf16_kernel<float, split, 128, 128> (3 BF16 MFMAs a product)"}[dom],
                     "achieved": round(achieved, 3), "peak": PEAK[dom], "unit": "TFLOP/s",
-                    "frac": round(achieved / PEAK[dom], 4), "traffic": profiled_traffic(wl, n),
+                    "frac": round(achieved / PEAK[dom], 4), "traffic": profiled_traffic(wl, n, dom),
                     "launches_per_step": cnt, "avg_launch_us": round(ms * 1e3 / cnt, 2)}
 
     result = {"metric": "FP64-accurate solves/sec & TFLOP/s vs plain FP64 GPU; p50/p99 solve latency",
