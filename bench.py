@@ -66,7 +66,8 @@ def parse():
                    help="mixed workloads: the factor precision (bf16: BF16 tiles with split operands, GMRES-IR)")
     p.add_argument("--requests", type=int, default=400, help="serve: requests per rate point")
     p.add_argument("--count", type=int, default=20000, help="problems per family (batched workloads)")
-    p.add_argument("--autotune", action="store_true", help="batched workloads: autotune the kernel variant")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="batched workloads: keep the default kernel variant instead of timing all four")
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--nrhs", type=int, default=None)
     p.add_argument("--kappa", type=float, default=1e6)
@@ -264,7 +265,7 @@ def bench_batch(a, np, torch, lmx):
     torch.cuda.synchronize()
     log(f"[bench] {a.workload}: {len(shapes)} problems, A pool {na * 8 / 2**30:.2f} GiB, inputs ready in "
         f"{time.perf_counter() - t0:.1f} s")
-    flags = lmx.BATCH_GRAPH | (lmx.BATCH_AUTOTUNE if a.autotune else 0)
+    flags = lmx.BATCH_GRAPH | (0 if a.no_autotune else lmx.BATCH_AUTOTUNE)
     prec = lmx.PREC_F32 if mixed else lmx.PREC_F64
     plan = lmx.BatchPlan(probs, na, nb, 0, prec, flags)
     q = plan.query()
