@@ -34,15 +34,27 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(int nb, T *__restrict__ 
                                                         int32_t *__restrict__ info) {
     __shared__ T S[DIAG][DIAG + 1];
     const int i = threadIdx.x;
-    // coalesced loads through LDS, then lane i takes row i
-    for (int e = i; e < nb * nb; e += 64) {
-        const int r = e / nb, c = e % nb;
-        S[r][c] = (c <= r) ? A[(int64_t)r * lda + c] : T(0);
+    // coalesced loads through LDS (lane i reads column i of each row), then
+    // lane i takes row i.  The loads are unconditional, at addresses clamped
+    // into the block, and issued 16 at a time: a load under a per-lane
+    // condition waits for memory before the next one issues.
+    const T *Ai = A + min(i, nb - 1);
+    for (int r0 = 0; r0 < DIAG; r0 += 16) {
+        T v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = Ai[(int64_t)min(r0 + u, nb - 1) * lda];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(v[u]));
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int r = r0 + u;
+            S[r][i] = (r < nb && i <= r) ? v[u] : T(0);
+        }
     }
     __syncthreads();
     T a[DIAG];
 #pragma unroll
-    for (int c = 0; c < DIAG; ++c) a[c] = (i < nb && c < nb) ? S[i][c] : T(0);
+    for (int c = 0; c < DIAG; ++c) a[c] = S[i][c];
     // All 64 steps run (compile-time indices keep a[] in registers); rows
     // and columns from nb on start at zero, may turn to NaN, and are never
     // stored.
@@ -60,10 +72,8 @@ __global__ __launch_bounds__(64) void potrf_diag_kernel(int nb, T *__restrict__ 
 #pragma unroll
     for (int c = 0; c < DIAG; ++c) S[i][c] = a[c];
     __syncthreads();
-    for (int e = i; e < nb * nb; e += 64) {
-        const int r = e / nb, c = e % nb;
-        if (c <= r) A[(int64_t)r * lda + c] = S[r][c];
-    }
+    for (int r = 0; r < nb; ++r)
+        if (i <= r) A[(int64_t)r * lda + i] = S[r][i];
     if (i == 0 && bad != 0 && *info == 0) *info = bad;
 }
 

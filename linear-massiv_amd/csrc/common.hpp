@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <utility>
+#include <vector>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -66,20 +67,40 @@ struct Context {
     // threshold at its maximum), so an allocation after a host
     // synchronisation does not map memory again
     hipMemPool_t pool = nullptr;
+    // Freed scratch blocks by the stream they were last used on, for reuse
+    // on that stream with no HIP call (stream order makes the reuse safe),
+    // at most kScratchCacheBytes in all; and idle lookahead streams.
+    struct Block {
+        void *p;
+        size_t bytes;
+    };
+    std::mutex scratch_mu;
+    std::vector<std::pair<hipStream_t, std::vector<Block>>> scratch_free;
+    size_t scratch_cached = 0;
+    struct Aux {
+        hipStream_t aux;
+        hipEvent_t ready, done;
+    };
+    std::vector<Aux> aux_free;
 };
 
 // Context of the current device (initialised on first use).
 Context &ctx();
+// Context of device `dev`, or nullptr when it is not initialised.
+Context *context_of(int dev);
 hipStream_t resolve_stream(void *s);
 // Grow-only device staging buffer `slot` of at least `bytes` (host calls only,
 // under host_mu).
 void *staging(Context &c, int slot, size_t bytes);
 
-// Stream-ordered scratch: allocated on `s` from the current device's pool,
-// freed on the same stream when the object goes out of scope.
+// Stream-ordered scratch on `s`: a cached block last used on `s` when one
+// fits, else a new one from the current device's pool; returned to the cache
+// when the object goes out of scope.
 struct Scratch {
     void *p = nullptr;
     hipStream_t s = nullptr;
+    size_t bytes = 0;
+    int device = -1;
     Scratch(size_t bytes, hipStream_t st);
     ~Scratch();
     Scratch(const Scratch &) = delete;
@@ -91,10 +112,13 @@ struct Scratch {
 // A second stream and two events for a factorisation's lookahead: the
 // caller's stream records `ready` when a block's update may start, the
 // auxiliary stream waits for it, runs the update and records `done`.
-// Destroyed (stream-ordered) when the factorisation returns.
+// Taken from the device's idle set (created on first need) and returned to
+// it when the factorisation returns; the next user's work follows in the
+// stream's order.
 struct AuxStream {
     hipStream_t aux = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
+    int device = -1;
     AuxStream();
     ~AuxStream();
     AuxStream(const AuxStream &) = delete;

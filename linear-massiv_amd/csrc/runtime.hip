@@ -83,6 +83,12 @@ Context &ctx() {
     return *g_ctx[dev];
 }
 
+Context *context_of(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    return g_ctx[dev];
+}
+
 // NULL is HIP's default (legacy) stream, which orders against every other
 // blocking stream -- the stream torch uses unless told otherwise.
 hipStream_t resolve_stream(void *s) { return static_cast<hipStream_t>(s); }
@@ -103,22 +109,117 @@ void *staging(Context &c, int slot, size_t bytes) {
     return c.staging[slot];
 }
 
-Scratch::Scratch(size_t bytes, hipStream_t st) : s(st) {
-    if (bytes == 0) bytes = 256;
-    LMX_HIP(hipMallocFromPoolAsync(&p, bytes, ctx().pool, s));
+// Scratch sizes are rounded up to 1/8 of their power of two, so that a
+// freed block serves later requests of about the same size; a cached block
+// is taken for a request of at least 4/5 of its size.
+namespace {
+constexpr size_t kScratchCacheBytes = size_t(8) << 30;
+
+size_t scratch_round(size_t bytes) {
+    size_t p2 = 256;
+    while (p2 < bytes) p2 <<= 1;
+    const size_t g = std::max<size_t>(256, p2 / 8);
+    return (bytes + g - 1) / g * g;
+}
+
+// Frees every cached block (stream-ordered on the stream it was last used
+// on); the caller holds scratch_mu.
+void release_cached(Context &c) {
+    for (auto &e : c.scratch_free)
+        for (auto &b : e.second) (void)hipFreeAsync(b.p, e.first);
+    c.scratch_free.clear();
+    c.scratch_cached = 0;
+}
+}  // namespace
+
+Scratch::Scratch(size_t req, hipStream_t st) : s(st), bytes(scratch_round(req == 0 ? 256 : req)) {
+    Context &c = ctx();
+    // Under stream capture the allocation must be the graph's own (a cached
+    // block would be handed out again while the graph still uses it).
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LMX_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) device = c.device;
+    if (device >= 0) {
+        std::lock_guard<std::mutex> g(c.scratch_mu);
+        for (auto &e : c.scratch_free) {
+            if (e.first != s) continue;
+            auto &v = e.second;
+            int best = -1;
+            for (int i = 0; i < (int)v.size(); ++i)
+                if (v[i].bytes >= bytes && v[i].bytes / 5 * 4 <= bytes && (best < 0 || v[i].bytes < v[best].bytes))
+                    best = i;
+            if (best >= 0) {
+                p = v[best].p;
+                bytes = v[best].bytes;
+                c.scratch_cached -= bytes;
+                v[best] = v.back();
+                v.pop_back();
+                return;
+            }
+            break;
+        }
+    }
+    hipError_t e = hipMallocFromPoolAsync(&p, bytes, c.pool, s);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> g(c.scratch_mu);
+            release_cached(c);
+        }
+        e = hipMallocFromPoolAsync(&p, bytes, c.pool, s);
+    }
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw_hip(e, "hipMallocFromPoolAsync", __FILE__, __LINE__);
+    }
 }
 
 Scratch::~Scratch() {
-    if (p) (void)hipFreeAsync(p, s);
+    if (!p) return;
+    Context *c = device >= 0 ? context_of(device) : nullptr;
+    if (c) {
+        std::lock_guard<std::mutex> g(c->scratch_mu);
+        if (c->scratch_cached + bytes <= kScratchCacheBytes) {
+            for (auto &e : c->scratch_free)
+                if (e.first == s) {
+                    e.second.push_back({p, bytes});
+                    c->scratch_cached += bytes;
+                    return;
+                }
+            c->scratch_free.push_back({s, {{p, bytes}}});
+            c->scratch_cached += bytes;
+            return;
+        }
+    }
+    (void)hipFreeAsync(p, s);
 }
 
 AuxStream::AuxStream() {
+    Context &c = ctx();
+    device = c.device;
+    {
+        std::lock_guard<std::mutex> g(c.scratch_mu);
+        if (!c.aux_free.empty()) {
+            aux = c.aux_free.back().aux;
+            ready = c.aux_free.back().ready;
+            done = c.aux_free.back().done;
+            c.aux_free.pop_back();
+            return;
+        }
+    }
+    // Default priority, non-blocking with respect to the legacy stream.
     LMX_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
     LMX_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     LMX_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
 }
 
 AuxStream::~AuxStream() {
+    Context *c = context_of(device);
+    if (c && aux && ready && done) {
+        std::lock_guard<std::mutex> g(c->scratch_mu);
+        c->aux_free.push_back({aux, ready, done});
+        return;
+    }
     if (ready) (void)hipEventDestroy(ready);
     if (done) (void)hipEventDestroy(done);
     if (aux) (void)hipStreamDestroy(aux);
@@ -177,6 +278,19 @@ int lmx_finalize(void) {
             LMX_HIP(hipStreamDestroy(c->stream));
             // scratch may still be in use on callers' streams until now
             LMX_HIP(hipDeviceSynchronize());
+            // every stream is idle: the blocks go back to the pool on the
+            // legacy stream (the stream a block last served may be gone)
+            for (auto &e : c->scratch_free)
+                for (auto &b : e.second) LMX_HIP(hipFreeAsync(b.p, nullptr));
+            LMX_HIP(hipDeviceSynchronize());
+            c->scratch_free.clear();
+            c->scratch_cached = 0;
+            for (auto &a : c->aux_free) {
+                LMX_HIP(hipEventDestroy(a.ready));
+                LMX_HIP(hipEventDestroy(a.done));
+                LMX_HIP(hipStreamDestroy(a.aux));
+            }
+            c->aux_free.clear();
             if (c->pool) LMX_HIP(hipMemPoolDestroy(c->pool));
             delete c;
             g_ctx[d] = nullptr;

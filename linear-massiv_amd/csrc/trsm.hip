@@ -32,27 +32,41 @@ constexpr int TB = 64;
 // is lower), so one forward solve serves both.  Thread t reads column t of
 // each stored row i -- coalesced -- which is M[i][t], or M[t][i] for a
 // transposed operand.  Rows of Ms are 16-byte aligned so the solve reads
-// four coefficients per access.
+// four coefficients per access.  The loads are unconditional, at addresses
+// clamped into the nb x nb block, and issued 16 at a time before the
+// triangle is selected with no branch: a load left under a per-lane
+// condition waits for memory before the next one issues.
 constexpr int ldm_of(int bytes) { return TB + 16 / bytes; }
 template <class T>
 __device__ inline void stage_block(T (*Ms)[ldm_of(sizeof(T))], const T *A, int64_t lda, int nb, int t, bool mt,
                                    bool mlower, bool unit, int S) {
+    if (nb <= 0) {
 #pragma unroll 16
-    for (int i = 0; i < TB; ++i) {
-        const int r = mt ? t : i, c = mt ? i : t;
-        if (r >= S || c >= S) continue;
-        T v;
-        if (r < nb && c < nb) {
-            const bool in = mlower ? c <= r : c >= r;
-            v = (in && !(unit && r == c)) ? A[(int64_t)i * lda + t] : T(0);
-            if (unit && r == c) v = T(1);
-        } else {
-            v = r == c ? T(1) : T(0);
+        for (int i = 0; i < TB; ++i) {
+            if (i >= S || t >= S) continue;
+            Ms[i][t] = i == t ? T(1) : T(0);
         }
-        if (mlower)
-            Ms[c][r] = v;
-        else
-            Ms[S - 1 - c][S - 1 - r] = v;
+        return;
+    }
+    const T *At = A + min(t, nb - 1);
+    for (int i0 = 0; i0 < TB; i0 += 16) {
+        T a[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] = At[(int64_t)min(i0 + u, nb - 1) * lda];
+        // keeps the compiler from sinking each load into the branch that
+        // uses it: all 16 are in flight before the first wait
+#pragma unroll
+        for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(a[u]));
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int i = i0 + u;
+            const int r = mt ? t : i, c = mt ? i : t;
+            const bool inblk = r < nb && c < nb, in = mlower ? c <= r : c >= r;
+            T v = inblk && in ? a[u] : T(0);
+            v = r == c && (unit || !inblk) ? T(1) : v;
+            T *dst = mlower ? &Ms[c][r] : &Ms[S - 1 - c][S - 1 - r];
+            if (r < S && c < S) *dst = v;
+        }
     }
 }
 
@@ -74,23 +88,37 @@ __device__ __forceinline__ void solve_regs(const T (*Ms)[ldm_of(sizeof(T))], T (
 
 // M x = b for one diagonal block (order nb <= S <= 64) and the lane's
 // right-hand-side vector, element r at p[r rs]: the vector is held in
-// registers (reversed for an upper M), the block in LDS.  The results go
-// back through the lane's own LDS column: a per-row guard on stores from
+// registers (reversed for an upper M), the block in LDS.  A lane that is
+// not live points p at a valid vector and stores nothing; `ident` takes
+// e_t, the lane's unit vector, in place of the loads.  The results go back
+// through the lane's own LDS column: a per-row guard on stores from
 // registers costs the compiler its register allocation.
 template <class T, int S>
 __device__ __forceinline__ void leaf_body(int nb, bool live, const T *__restrict__ A, int64_t lda, T *p, int64_t rs,
-                                          bool mt, bool mlower, bool unit) {
+                                          bool mt, bool mlower, bool unit, bool ident) {
     __shared__ __attribute__((aligned(16))) T Ms[TB][ldm_of(sizeof(T))];
     __shared__ T Xs[TB][TB + 1];
     const int t = threadIdx.x;
-    stage_block<T>(Ms, A, lda, nb, t, mt, mlower, unit, S);
     T x[TB];
+    if (ident) {
 #pragma unroll
-    for (int i = 0; i < TB; ++i) {
-        const int r = mlower ? i : S - 1 - i;  // register i holds element r
-        x[i] = T(0);
-        if (i < S && live && r < nb) x[i] = p[r * rs];
+        for (int i = 0; i < TB; ++i) {
+            const int r = mlower ? i : S - 1 - i;  // register i holds element r
+            x[i] = (i < S && r < nb && r == t) ? T(1) : T(0);
+        }
+    } else {
+        const int nbc = max(nb, 1);
+#pragma unroll
+        for (int i = 0; i < TB; ++i) {
+            const int r = mlower ? i : S - 1 - i;
+            x[i] = T(0);
+            if (i < S) {
+                const T a = p[(int64_t)min(r, nbc - 1) * rs];
+                x[i] = r < nb ? a : T(0);
+            }
+        }
     }
+    stage_block<T>(Ms, A, lda, nb, t, mt, mlower, unit, S);
     __syncthreads();
     solve_regs<T, S>(Ms, x);
 #pragma unroll
@@ -105,9 +133,9 @@ template <class T, int S>
 __global__ __launch_bounds__(64) void trsm_leaf_kernel(int nb, int nvec, const T *__restrict__ A, int64_t lda,
                                                        T *__restrict__ B, int64_t ldb, int mt, int mlower, int unit,
                                                        int right) {
-    const int v = blockIdx.x * 64 + threadIdx.x;
-    T *p = right ? B + (int64_t)v * ldb : B + v;
-    leaf_body<T, S>(nb, v < nvec, A, lda, p, right ? 1 : ldb, mt != 0, mlower != 0, unit != 0);
+    const int v = blockIdx.x * 64 + threadIdx.x, vc = min(v, nvec - 1);
+    T *p = right ? B + (int64_t)vc * ldb : B + vc;
+    leaf_body<T, S>(nb, v < nvec, A, lda, p, right ? 1 : ldb, mt != 0, mlower != 0, unit != 0, false);
 }
 
 // W_b = inverse of diagonal block b of M (order min(64, n - 64 b), padded
@@ -123,7 +151,7 @@ __global__ __launch_bounds__(64) void trtri_blocks_kernel(int n, const T *__rest
     const int per = wld / TB, zk = b / per, i0 = b % per;
     T *Wb = W + (int64_t)zk * wld * wld + (int64_t)i0 * TB * wld + i0 * TB;
     leaf_body<T, TB>(min(TB, n - r0), true, A + (int64_t)r0 * lda + r0, lda, Wb + threadIdx.x, wld, mt != 0,
-                     mlower != 0, unit != 0);
+                     mlower != 0, unit != 0, true);
 }
 
 // ---------------------------------------------------------------- block inverses
