@@ -1,10 +1,13 @@
-// Lower Cholesky factorisation (right-looking, blocked) and solve, for
-// row-major matrices.  Only the lower triangle is read or written.
+// Lower Cholesky factorisation (right-looking, blocked, with lookahead) and
+// solve, for row-major matrices.  Only the lower triangle is read or written.
 //
-// potrf_blocked(nb): for each diagonal block of width nb:
-//   - factor the block (recursively with nb = 64, then the one-workgroup
-//     LDS kernel potrf_diag_kernel for blocks of order <= 64);
-//   - L21 = A21 L11^{-T} (right-side blocked TRSM);
+// For each block column of width 128:
+//   - the diagonal block and its inverse (diag_block): the 64 x 64 kernel
+//     potrf_diag_kernel factors each half and inverts it in the same pass;
+//     between the halves, L10 = A10 L00^{-T} and A11 -= L10 L10^T are small
+//     MFMA products, and the inverse's off-diagonal block -L11^{-1} L10
+//     L00^{-1} two more;
+//   - L21 = A21 L11^{-T}: one GEMM with the block's inverse;
 //   - A22 -= L21 L21^T on the lower-triangle-only MFMA SYRK.
 // The reference's left-looking panelCholFactor (src/.../Solve/Cholesky.hs:
 // 279-321) computes the same factor; the right-looking order keeps the
@@ -22,59 +25,138 @@ namespace {
 
 constexpr int DIAG = 64;
 
-// In-place lower Cholesky of an nb x nb block (nb <= 64): one wave, lane i
-// holding row i in registers.  Step j takes the diagonal and each l_cj
-// from the lane that holds it (v_readlane into a scalar register), so a
-// step has no barrier: l_jj = sqrt(a_jj), l_ij = a_ij / l_jj (a division,
-// as rawCholColumnSIMD), then a_ic -= l_ij l_cj for c > j, in ascending j
-// for every element.  Lanes also update their row's columns above the
-// diagonal; those registers are never read or stored.
+// In-place lower Cholesky of an nb x nb block (nb <= 64) by four waves:
+// lane i holds row i, wave w its columns 16w .. 16w+15.  Step j: the wave
+// that holds column j takes l_jj = sqrt(a_jj) from lane j (v_readlane),
+// divides its column by it (l_ij = a_ij / l_jj, a division, as
+// rawCholColumnSIMD), publishes the column in LDS and updates its own
+// columns after j; after one barrier the waves holding later columns apply
+// a_ic -= l_ij l_cj to all of theirs.  Every element takes its terms in
+// ascending j.  Lanes also update their row's columns above the diagonal;
+// those values are never stored.
+//
+// The same loop builds the inverse X = L^{-1}, one step behind: row k is
+// final at step k + 1 (x_k = (e_k - sum_{m<k} l_km x_m) / l_kk, the terms
+// in ascending m -- the order of the 64 x 64 inverse kernel's forward
+// substitution, with the identity beyond nb).  Lane i holds column i of X;
+// the rows 16g .. 16g+15 sit in wave (g + 1) mod 4, so that the division
+// that finishes row k runs beside the next column's square root, not after
+// it.  Every wave applies each finished row of X to all of its rows (zero
+// multipliers for the rows before it), with no branch.  X^T, upper, goes to WT (row stride ldw), so that B L^{-T} is a
+// plain product B WT; Z, when given, is a 64 x 64 block (row stride ldw)
+// set to zero.
 template <class T>
-__global__ __launch_bounds__(64) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
-                                                        int32_t *__restrict__ info) {
+__global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
+                                                         int32_t *__restrict__ info, T *__restrict__ WT,
+                                                         int64_t ldw, T *__restrict__ Z) {
     __shared__ T S[DIAG][DIAG + 1];
-    const int i = threadIdx.x;
-    // coalesced loads through LDS (lane i reads column i of each row), then
-    // lane i takes row i.  The loads are unconditional, at addresses clamped
-    // into the block, and issued 16 at a time: a load under a per-lane
-    // condition waits for memory before the next one issues.
-    const T *Ai = A + min(i, nb - 1);
-    for (int r0 = 0; r0 < DIAG; r0 += 16) {
+    __shared__ __attribute__((aligned(16))) T C[DIAG][DIAG];       // C[j][i] = l_ij
+    __shared__ __attribute__((aligned(16))) T R[DIAG][DIAG + 1];   // R[k][t] = x_kt
+    __shared__ int sbad[4];
+    // the wave index, uniform in a scalar register
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), i = threadIdx.x & 63;
+    const int g = (w + 3) & 3;  // the group of rows of X this wave holds
+    // coalesced loads through LDS (lane i reads column i of the wave's 16
+    // rows), unconditional at addresses clamped into the block: a load under
+    // a per-lane condition waits for memory before the next one issues
+    {
+        const T *Ai = A + min(i, nb - 1);
         T v[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = Ai[(int64_t)min(r0 + u, nb - 1) * lda];
+        for (int u = 0; u < 16; ++u) v[u] = Ai[(int64_t)min(16 * w + u, nb - 1) * lda];
 #pragma unroll
         for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(v[u]));
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int r = r0 + u;
+            const int r = 16 * w + u;
             S[r][i] = (r < nb && i <= r) ? v[u] : T(0);
         }
     }
     __syncthreads();
-    T a[DIAG];
+    T a[16], x[16];
 #pragma unroll
-    for (int c = 0; c < DIAG; ++c) a[c] = S[i][c];
-    // All 64 steps run (compile-time indices keep a[] in registers); rows
-    // and columns from nb on start at zero, may turn to NaN, and are never
-    // stored.
-    int bad = 0;
-#pragma unroll
-    for (int j = 0; j < DIAG; ++j) {
-        const T d = lane_value(a[j], j);
-        const T sjj = sqrt(d);
-        if (j < nb && !(d > T(0)) && bad == 0) bad = col0 + j + 1;
-        a[j] = i == j ? sjj : a[j] / sjj;  // rows above j: unused registers
-#pragma unroll
-        for (int c = j + 1; c < DIAG; ++c) a[c] = fma(-a[j], lane_value(a[j], c), a[c]);
+    for (int u = 0; u < 16; ++u) {
+        a[u] = S[i][16 * w + u];
+        x[u] = 16 * g + u == i ? T(1) : T(0);
     }
+    // The steps run in four groups of 16, each unrolled, so that every
+    // register index is a compile-time constant.  All 64 steps run; rows
+    // and columns from nb on start at zero, may turn to NaN, and are never
+    // stored.  A published column is the identity's beyond nb (no real
+    // entry depends on those), which gives the inverse its identity padding.
+    int bad = 0;
+    for (int jo = 0; jo < 4; ++jo) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = 16 * jo + jj;
+            if (w == jo) {
+                const T d = lane_value(a[jj], j);
+                const T sjj = sqrt(d);
+                if (j < nb && !(d > T(0)) && bad == 0) bad = col0 + j + 1;
+                a[jj] = i == j ? sjj : a[jj] / sjj;
+                const T li = (i >= j && i < nb && j < nb) ? a[jj] : (i == j ? T(1) : T(0));
+                C[j][i] = li;
+                // this wave's columns after j (its own LDS writes are
+                // visible to its reads without a barrier)
+                T cj[16];
+#pragma unroll
+                for (int u = jj + 1; u < 16; ++u) cj[u] = C[j][16 * jo + u];
+#pragma unroll
+                for (int u = jj + 1; u < 16; ++u) a[u] = fma(-li, cj[u], a[u]);
+            }
+            // row k of X, one step behind
+            const int k = j - 1, ko = jj == 0 ? jo - 1 : jo, kk = (jj + 15) & 15;
+            if (j > 0 && w == ((ko + 1) & 3)) {
+                x[kk] = x[kk] / C[k][k];
+                R[k][i] = x[kk];
+            }
+            __syncthreads();
+            // the later groups: every column (row of X) of the wave, reads
+            // issued together
+            if (w > jo) {
+                const T li = C[j][i];
+                T cj[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) cj[u] = C[j][16 * w + u];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) a[u] = fma(-li, cj[u], a[u]);
+            }
+            // every row of X after k; a published column is zero above its
+            // diagonal, so the rows before k are left as they are (the
+            // diagonal term itself is masked)
+            if (j > 0) {
+                const T xk = R[k][i];
+                T ck[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) ck[u] = C[k][16 * g + u];
+                ck[kk] = g == ko ? T(0) : ck[kk];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = fma(-ck[u], xk, x[u]);
+            }
+        }
+    }
+    if (w == 0) {  // row 63 of X (group 3)
+        x[15] = x[15] / C[DIAG - 1][DIAG - 1];
+        R[DIAG - 1][i] = x[15];
+    }
+    if (i == 0) sbad[w] = bad;
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < DIAG; ++c) S[i][c] = a[c];
+    for (int u = 0; u < 16; ++u) S[i][16 * w + u] = a[u];
     __syncthreads();
-    for (int r = 0; r < nb; ++r)
-        if (i <= r) A[(int64_t)r * lda + i] = S[r][i];
-    if (i == 0 && bad != 0 && *info == 0) *info = bad;
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+        const int r = 16 * w + u;
+        if (r < nb && i <= r) A[(int64_t)r * lda + i] = S[r][i];
+        WT[(int64_t)r * ldw + i] = R[i][r];  // WT[r][i] = x_ir
+        if (Z) Z[(int64_t)r * ldw + i] = T(0);
+    }
+    if (threadIdx.x == 0) {
+        int first = 0;
+        for (int q = 0; q < 4; ++q)
+            if (sbad[q] != 0 && (first == 0 || sbad[q] < first)) first = sbad[q];
+        if (first != 0 && *info == 0) *info = first;
+    }
 }
 
 // C -= A B^T (lower tiles only with `lower`): FP64 / FP32 MFMA, or the BF16
@@ -95,26 +177,53 @@ void chol_update(hipStream_t s, bool bf16, bool lower, int64_t m, int64_t n, int
 }
 
 template <class T>
-void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, int32_t *d_info, int64_t nb,
-                   bool bf16) {
-    for (int64_t k0 = 0; k0 < n; k0 += nb) {
-        const int64_t kb = std::min(nb, n - k0);
-        T *A11 = A + k0 * lda + k0;
-        if (kb <= DIAG) {
-            TimedRegion tr("potrf_diag", s);
-            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, s, (int)kb, A11, lda,
-                               (int)(col0 + k0), d_info);
-            LMX_LAUNCH_CHECK();
-        } else {
-            potrf_blocked<T>(s, kb, A11, lda, col0 + k0, d_info, DIAG, bf16);
-        }
-        const int64_t rest = n - k0 - kb;
-        if (rest > 0) {
-            T *A21 = A11 + kb * lda;
-            trsm_right<T>(s, true, true, false, rest, kb, A11, lda, A21, lda);
-            chol_update<T>(s, bf16, true, rest, rest, kb, A21, lda, A21, A21 + kb);
-        }
-    }
+__global__ __launch_bounds__(256) void copy_block_kernel(const T *__restrict__ in, int64_t ldi, T *__restrict__ out,
+                                                         int64_t ldo, int64_t rows, int cols) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= rows * cols) return;
+    const int64_t r = e / cols;
+    const int c = (int)(e - r * cols);
+    out[r * ldo + c] = in[r * ldi + c];
+}
+
+template <class T>
+void copy_block(hipStream_t s, const T *in, int64_t ldi, T *out, int64_t ldo, int64_t rows, int64_t cols) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL(copy_block_kernel<T>, dim3((unsigned)cdiv(rows * cols, 256)), dim3(256), 0, s, in, ldi, out,
+                       ldo, rows, (int)cols);
+    LMX_LAUNCH_CHECK();
+}
+
+// Factors a 64 x 64 (or smaller) diagonal block and writes its L^{-T} to WT.
+template <class T>
+void diag_inv(hipStream_t s, int64_t nb, T *A, int64_t lda, int64_t col0, int32_t *d_info, T *WT, int64_t ldw,
+              T *Z) {
+    TimedRegion tr("potrf_diag", s);
+    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)nb, A, lda, (int)col0, d_info, WT, ldw,
+                       Z);
+    LMX_LAUNCH_CHECK();
+}
+
+constexpr int BLK = 2 * DIAG;  // block column width; WT's row stride
+
+// The diagonal block A11 (order kb <= 128) in place, and L11^{-T} (upper,
+// kb x kb) in WT.  With L = [[L00, 0], [L10, L11]] (halves of 64 and kb -
+// 64): L^{-T} = [[L00^{-T}, -L00^{-T} L10^T L11^{-T}], [0, L11^{-T}]], the
+// off-diagonal block as the inverse's doubling step forms it.  tmp holds
+// 64 x 64.
+template <class T>
+void diag_block(hipStream_t s, bool bf16, int64_t kb, T *A11, int64_t lda, int64_t col0, int32_t *d_info, T *WT,
+                T *tmp) {
+    const int64_t k0 = std::min<int64_t>(kb, DIAG), k1 = kb - k0;
+    diag_inv(s, k0, A11, lda, col0, d_info, WT, BLK, (T *)nullptr);
+    if (k1 == 0) return;
+    T *A10 = A11 + DIAG * lda, *A11b = A10 + DIAG, *WT11 = WT + DIAG * BLK + DIAG;
+    gemm(s, false, false, k1, DIAG, DIAG, T(1), A10, lda, WT, BLK, T(0), tmp, DIAG);
+    copy_block(s, tmp, DIAG, A10, lda, k1, DIAG);
+    chol_update<T>(s, bf16, true, k1, k1, DIAG, A10, lda, A10, A11b);
+    diag_inv(s, k1, A11b, lda, col0 + DIAG, d_info, WT11, BLK, WT + DIAG * BLK);
+    gemm(s, true, false, DIAG, k1, k1, T(1), A10, lda, WT11, BLK, T(0), tmp, DIAG);
+    gemm(s, false, false, DIAG, k1, DIAG, T(-1), WT, BLK, tmp, DIAG, T(0), WT + DIAG, BLK);
 }
 
 // The top level with lookahead: the caller's stream factors block column
@@ -126,23 +235,26 @@ void potrf_blocked(hipStream_t s, int64_t n, T *A, int64_t lda, int64_t col0, in
 //               record done(b)
 // While they overlap the streams write disjoint columns (b + 1's against
 // those beyond it), and the auxiliary stream only reads block column b.
+// The panel below a diagonal block is L21 = A21 L11^{-T}: one GEMM into
+// scratch and a copy back.
 template <class T>
-void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, int64_t nb, bool bf16) {
+void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, bool bf16) {
+    constexpr int64_t nb = BLK;
     AuxStream la;
     bool pending = false;
+    Scratch sw((size_t)BLK * BLK * sizeof(T), s), st((size_t)DIAG * DIAG * sizeof(T), s);
+    Scratch sx((size_t)std::max<int64_t>(n - nb, 1) * nb * sizeof(T), s);
+    T *WT = sw.as<T>(), *tmp = st.as<T>(), *X = sx.as<T>();
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         const int64_t kb = std::min(nb, n - k0);
         T *A11 = A + k0 * lda + k0;
-        if (kb <= DIAG) {
-            TimedRegion tr("potrf_diag", s);
-            hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(64), 0, s, (int)kb, A11, lda, (int)k0, d_info);
-            LMX_LAUNCH_CHECK();
-        } else {
-            potrf_blocked<T>(s, kb, A11, lda, k0, d_info, DIAG, bf16);
-        }
         const int64_t rest = n - k0 - kb;
         T *A21 = A11 + kb * lda;
-        if (rest > 0) trsm_right<T>(s, true, true, false, rest, kb, A11, lda, A21, lda);
+        diag_block<T>(s, bf16, kb, A11, lda, k0, d_info, WT, tmp);
+        if (rest > 0) {
+            gemm(s, false, false, rest, kb, kb, T(1), A21, lda, WT, BLK, T(0), X, kb);
+            copy_block(s, X, kb, A21, lda, rest, kb);
+        }
         if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
         const int64_t narrow = std::min(nb, rest);
         if (rest > 0) {
@@ -172,7 +284,7 @@ void potrf(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_info, bool bf
     if (n == 0) return;
     TimedRegion trall("potrf", s, (double)n * (double)n * (double)n / 3.0);
     if (bf16 && sizeof(T) != 4) fail_arg("potrf: BF16 updates are for the FP32-stored factor");
-    potrf_lookahead<T>(s, n, A, lda, d_info, 128, bf16);
+    potrf_lookahead<T>(s, n, A, lda, d_info, bf16);
 }
 
 template <class T>
