@@ -1,12 +1,12 @@
 // Lower Cholesky factorisation (right-looking, blocked, with lookahead) and
 // solve, for row-major matrices.  Only the lower triangle is read or written.
 //
-// For each block column of width 128:
-//   - the diagonal block and its inverse (diag_block): the 64 x 64 kernel
-//     potrf_diag_kernel factors each half and inverts it in the same pass;
-//     between the halves, L10 = A10 L00^{-T} and A11 -= L10 L10^T are small
-//     MFMA products, and the inverse's off-diagonal block -L11^{-1} L10
-//     L00^{-1} two more;
+// For each block column of width 256:
+//   - the diagonal block and its inverse (diag_block), by halves down to
+//     64 x 64, which potrf_diag_kernel factors and inverts in one pass;
+//     between two halves, L10 = A10 L00^{-T} and A11 -= L10 L10^T are MFMA
+//     products, and the inverse's off-diagonal block -L11^{-1} L10 L00^{-1}
+//     two more;
 //   - L21 = A21 L11^{-T}: one GEMM with the block's inverse;
 //   - A22 -= L21 L21^T on the lower-triangle-only MFMA SYRK.
 // The reference's left-looking panelCholFactor (src/.../Solve/Cholesky.hs:
@@ -42,13 +42,12 @@ constexpr int DIAG = 64;
 // the rows 16g .. 16g+15 sit in wave (g + 1) mod 4, so that the division
 // that finishes row k runs beside the next column's square root, not after
 // it.  Every wave applies each finished row of X to all of its rows (zero
-// multipliers for the rows before it), with no branch.  X^T, upper, goes to WT (row stride ldw), so that B L^{-T} is a
-// plain product B WT; Z, when given, is a 64 x 64 block (row stride ldw)
-// set to zero.
+// multipliers for the rows before it), with no branch.  X^T, upper, goes
+// to WT (row stride ldw), so that B L^{-T} is a plain product B WT.
 template <class T>
 __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
                                                          int32_t *__restrict__ info, T *__restrict__ WT,
-                                                         int64_t ldw, T *__restrict__ Z) {
+                                                         int64_t ldw) {
     __shared__ T S[DIAG][DIAG + 1];
     __shared__ __attribute__((aligned(16))) T C[DIAG][DIAG];       // C[j][i] = l_ij
     __shared__ __attribute__((aligned(16))) T R[DIAG][DIAG + 1];   // R[k][t] = x_kt
@@ -149,7 +148,6 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
         const int r = 16 * w + u;
         if (r < nb && i <= r) A[(int64_t)r * lda + i] = S[r][i];
         WT[(int64_t)r * ldw + i] = R[i][r];  // WT[r][i] = x_ir
-        if (Z) Z[(int64_t)r * ldw + i] = T(0);
     }
     if (threadIdx.x == 0) {
         int first = 0;
@@ -196,34 +194,44 @@ void copy_block(hipStream_t s, const T *in, int64_t ldi, T *out, int64_t ldo, in
 
 // Factors a 64 x 64 (or smaller) diagonal block and writes its L^{-T} to WT.
 template <class T>
-void diag_inv(hipStream_t s, int64_t nb, T *A, int64_t lda, int64_t col0, int32_t *d_info, T *WT, int64_t ldw,
-              T *Z) {
+void diag_inv(hipStream_t s, int64_t nb, T *A, int64_t lda, int64_t col0, int32_t *d_info, T *WT, int64_t ldw) {
     TimedRegion tr("potrf_diag", s);
-    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)nb, A, lda, (int)col0, d_info, WT, ldw,
-                       Z);
+    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)nb, A, lda, (int)col0, d_info, WT, ldw);
     LMX_LAUNCH_CHECK();
 }
 
-constexpr int BLK = 2 * DIAG;  // block column width; WT's row stride
+template <class T>
+__global__ __launch_bounds__(256) void fill_zero_kernel(T *__restrict__ p, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) p[e] = T(0);
+}
 
-// The diagonal block A11 (order kb <= 128) in place, and L11^{-T} (upper,
-// kb x kb) in WT.  With L = [[L00, 0], [L10, L11]] (halves of 64 and kb -
-// 64): L^{-T} = [[L00^{-T}, -L00^{-T} L10^T L11^{-T}], [0, L11^{-T}]], the
-// off-diagonal block as the inverse's doubling step forms it.  tmp holds
-// 64 x 64.
+constexpr int BLK = 4 * DIAG;  // block column width (64 times a power of two); WT's row stride
+
+// The diagonal block A11 (order kb <= BLK) in place, and L11^{-T} (upper,
+// kb x kb) in WT, whose part below the diagonal blocks the caller has set
+// to zero.  Order > 64: halves of h (64 times a power of two, h < kb <=
+// 2h) and kb - h; with L = [[L00, 0], [L10, L11]], L^{-T} = [[L00^{-T},
+// -L00^{-T} L10^T L11^{-T}], [0, L11^{-T}]], the off-diagonal block as the
+// inverse's doubling step forms it.  tmp holds (BLK / 2)^2.
 template <class T>
 void diag_block(hipStream_t s, bool bf16, int64_t kb, T *A11, int64_t lda, int64_t col0, int32_t *d_info, T *WT,
                 T *tmp) {
-    const int64_t k0 = std::min<int64_t>(kb, DIAG), k1 = kb - k0;
-    diag_inv(s, k0, A11, lda, col0, d_info, WT, BLK, (T *)nullptr);
-    if (k1 == 0) return;
-    T *A10 = A11 + DIAG * lda, *A11b = A10 + DIAG, *WT11 = WT + DIAG * BLK + DIAG;
-    gemm(s, false, false, k1, DIAG, DIAG, T(1), A10, lda, WT, BLK, T(0), tmp, DIAG);
-    copy_block(s, tmp, DIAG, A10, lda, k1, DIAG);
-    chol_update<T>(s, bf16, true, k1, k1, DIAG, A10, lda, A10, A11b);
-    diag_inv(s, k1, A11b, lda, col0 + DIAG, d_info, WT11, BLK, WT + DIAG * BLK);
-    gemm(s, true, false, DIAG, k1, k1, T(1), A10, lda, WT11, BLK, T(0), tmp, DIAG);
-    gemm(s, false, false, DIAG, k1, DIAG, T(-1), WT, BLK, tmp, DIAG, T(0), WT + DIAG, BLK);
+    if (kb <= DIAG) {
+        diag_inv(s, kb, A11, lda, col0, d_info, WT, BLK);
+        return;
+    }
+    int64_t h = DIAG;
+    while (2 * h < kb) h *= 2;
+    const int64_t k1 = kb - h;
+    diag_block<T>(s, bf16, h, A11, lda, col0, d_info, WT, tmp);
+    T *A10 = A11 + h * lda, *A11b = A10 + h, *WT11 = WT + h * BLK + h;
+    gemm(s, false, false, k1, h, h, T(1), A10, lda, WT, BLK, T(0), tmp, h);  // L10 = A10 L00^{-T}
+    copy_block(s, tmp, h, A10, lda, k1, h);
+    chol_update<T>(s, bf16, true, k1, k1, h, A10, lda, A10, A11b);
+    diag_block<T>(s, bf16, k1, A11b, lda, col0 + h, d_info, WT11, tmp);
+    gemm(s, true, false, h, k1, k1, T(1), A10, lda, WT11, BLK, T(0), tmp, k1);
+    gemm(s, false, false, h, k1, h, T(-1), WT, BLK, tmp, k1, T(0), WT + h, BLK);
 }
 
 // The top level with lookahead: the caller's stream factors block column
@@ -242,9 +250,14 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
     constexpr int64_t nb = BLK;
     AuxStream la;
     bool pending = false;
-    Scratch sw((size_t)BLK * BLK * sizeof(T), s), st((size_t)DIAG * DIAG * sizeof(T), s);
+    Scratch sw((size_t)BLK * BLK * sizeof(T), s), st((size_t)(BLK / 2) * (BLK / 2) * sizeof(T), s);
     Scratch sx((size_t)std::max<int64_t>(n - nb, 1) * nb * sizeof(T), s);
     T *WT = sw.as<T>(), *tmp = st.as<T>(), *X = sx.as<T>();
+    // the inverse's blocks below its diagonal blocks stay zero: nothing
+    // writes there
+    hipLaunchKernelGGL(fill_zero_kernel<T>, dim3((unsigned)cdiv(BLK * BLK, 256)), dim3(256), 0, s, WT,
+                       (int64_t)BLK * BLK);
+    LMX_LAUNCH_CHECK();
     for (int64_t k0 = 0; k0 < n; k0 += nb) {
         const int64_t kb = std::min(nb, n - k0);
         T *A11 = A + k0 * lda + k0;
