@@ -115,9 +115,13 @@ __device__ inline int64_t ix(int r, int c, int n) { return (int64_t)r * n + c; }
 
 // Li (unit lower inverse) and Ui (upper inverse) of the w x w diagonal block
 // held in C rows 0..w-1 (LU), padded to NB x NB with the identity; written
-// to LDS and to the slot (row-major NB x NB).
+// to LDS and to the slot (row-major NB x NB).  Lane c of wave 0 builds
+// column c of Li, lane c of wave 1 column c of Ui (separate waves, so the
+// two run side by side); each element is a dot product whose loads are
+// unrolled by 8 (one LDS round trip for eight terms, not one each).
 template <class F, int NB, int NT>
-__device__ void lu_tri_inverses(const F *C, int w, F *Li, F *Ui, F *gL, F *gU) {
+__device__ void lu_tri_inverses_lds(const F *C, int w, F *Li, F *Ui, F *gL, F *gU) {
+    static_assert(NT >= 128, "the two inverses run in waves 0 and 1");
     constexpr int LDC = NB + 1;
     const int t = threadIdx.x;
     if (t < NB) {
@@ -128,19 +132,21 @@ __device__ void lu_tri_inverses(const F *C, int w, F *Li, F *Ui, F *gL, F *gU) {
             else if (i == c) v = F(1);
             else if (i < w && c < w) {
                 F s = F(0);
+#pragma unroll 8
                 for (int k = c; k < i; ++k) s = fma(C[i * LDC + k], Li[k * LDC + c], s);
                 v = -s;
             } else v = F(0);
             Li[i * LDC + c] = v;
         }
-    } else if (t < 2 * NB) {
-        const int c = t - NB;
+    } else if (t >= 64 && t < 64 + NB) {
+        const int c = t - 64;
         for (int i = NB - 1; i >= 0; --i) {
             F v;
             if (i > c) v = F(0);
             else if (c >= w) v = (i == c) ? F(1) : F(0);
             else {
                 F s = (i == c) ? F(1) : F(0);
+#pragma unroll 8
                 for (int k = i + 1; k <= c; ++k) s = fma(-C[i * LDC + k], Ui[k * LDC + c], s);
                 v = s / C[i * LDC + i];
             }
@@ -155,9 +161,10 @@ __device__ void lu_tri_inverses(const F *C, int w, F *Li, F *Ui, F *gL, F *gU) {
     }
 }
 
-// Li = inverse of the non-unit lower w x w block in C (Cholesky).
+// Li = inverse of the non-unit lower w x w block in C (Cholesky); the dot
+// products as in lu_tri_inverses.
 template <class F, int NB, int NT>
-__device__ void chol_tri_inverse(const F *C, int w, F *Li, F *gL) {
+__device__ void chol_tri_inverse_lds(const F *C, int w, F *Li, F *gL) {
     constexpr int LDC = NB + 1;
     const int t = threadIdx.x;
     if (t < NB) {
@@ -169,6 +176,7 @@ __device__ void chol_tri_inverse(const F *C, int w, F *Li, F *gL) {
             else if (i == c) v = F(1) / C[c * LDC + c];
             else if (i < w) {
                 F s = F(0);
+#pragma unroll 8
                 for (int k = c; k < i; ++k) s = fma(C[i * LDC + k], Li[k * LDC + c], s);
                 v = -s / C[i * LDC + i];
             } else v = F(0);
@@ -177,6 +185,124 @@ __device__ void chol_tri_inverse(const F *C, int w, F *Li, F *gL) {
     }
     __syncthreads();
     for (int e = t; e < NB * NB; e += NT) gL[e] = Li[(e / NB) * LDC + e % NB];
+}
+
+// The same inverses with each column held in registers by its lane (lane c
+// of wave 0 for Li, of wave 1 for Ui), every element a dot product in
+// ascending k over the column's triangle (terms outside it are skipped);
+// the coefficients are broadcast LDS reads that do not depend on the
+// column, so they issue together.  Used for FP32 factors: with FP64 the
+// unrolled columns and their loads cost the kernel its registers.
+template <class F, int NB, int NT>
+__device__ void lu_tri_inverses_regs(const F *C, int w, F *Li, F *Ui, F *gL, F *gU) {
+    static_assert(NT >= 128, "the two inverses run in waves 0 and 1");
+    constexpr int LDC = NB + 1;
+    const int t = threadIdx.x;
+    if (t < NB) {
+        const int c = t;
+        F x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            // Li[i][c] = -sum_{k=c}^{i-1} L[i][k] Li[k][c]  (i > c, both < w)
+            F sum = F(0);
+#pragma unroll
+            for (int k = 0; k < i; ++k) {
+                const F a = C[i * LDC + k];
+                sum = k >= c ? fma(a, x[k], sum) : sum;
+            }
+            x[i] = i == c ? F(1) : ((i > c && i < w && c < w) ? -sum : F(0));
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) Li[i * LDC + c] = x[i];
+    } else if (t >= 64 && t < 64 + NB) {
+        const int c = t - 64;
+        F y[NB];
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            // Ui[i][c] = ((i == c) - sum_{k=i+1}^{c} U[i][k] Ui[k][c]) / U[i][i]
+            F sum = i == c ? F(1) : F(0);
+#pragma unroll
+            for (int k = i + 1; k < NB; ++k) {
+                const F a = C[i * LDC + k];
+                sum = k <= c ? fma(-a, y[k], sum) : sum;
+            }
+            const F q = sum / C[i * LDC + i];
+            y[i] = i > c ? F(0) : (c >= w ? (i == c ? F(1) : F(0)) : q);
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) Ui[i * LDC + c] = y[i];
+    }
+    __syncthreads();
+    for (int e = t; e < NB * NB; e += NT) {
+        const int i = e / NB, c = e % NB;
+        gL[e] = Li[i * LDC + c];
+        gU[e] = Ui[i * LDC + c];
+    }
+}
+
+template <class F, int NB, int NT>
+__device__ void chol_tri_inverse_regs(const F *C, int w, F *Li, F *gL) {
+    constexpr int LDC = NB + 1;
+    const int t = threadIdx.x;
+    if (t < NB) {
+        const int c = t;
+        F x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            // Li[i][c] = -sum_{k=c}^{i-1} L[i][k] Li[k][c] / L[i][i] (i > c),
+            // 1 / L[c][c] on the diagonal
+            F sum = F(0);
+#pragma unroll
+            for (int k = 0; k < i; ++k) {
+                const F a = C[i * LDC + k];
+                sum = k >= c ? fma(a, x[k], sum) : sum;
+            }
+            const F dv = C[i * LDC + i];
+            F v;
+            if (i < c)
+                v = F(0);
+            else if (c >= w)
+                v = i == c ? F(1) : F(0);
+            else if (i == c)
+                v = F(1) / dv;
+            else if (i < w)
+                v = -sum / dv;
+            else
+                v = F(0);
+            x[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) Li[i * LDC + c] = x[i];
+    }
+    __syncthreads();
+    for (int e = t; e < NB * NB; e += NT) gL[e] = Li[(e / NB) * LDC + e % NB];
+}
+
+template <class F, int NB, int NT>
+__device__ void lu_tri_inverses(const F *C, int w, F *Li, F *Ui, F *gL, F *gU) {
+    if constexpr (sizeof(F) == 4)
+        lu_tri_inverses_regs<F, NB, NT>(C, w, Li, Ui, gL, gU);
+    else
+        lu_tri_inverses_lds<F, NB, NT>(C, w, Li, Ui, gL, gU);
+}
+
+template <class F, int NB, int NT>
+__device__ void chol_tri_inverse(const F *C, int w, F *Li, F *gL) {
+    if constexpr (sizeof(F) == 4)
+        chol_tri_inverse_regs<F, NB, NT>(C, w, Li, gL);
+    else
+        chol_tri_inverse_lds<F, NB, NT>(C, w, Li, gL);
+}
+
+// cr[cc] -= l * cp[cc * S] for cc in [c1, c2): one row's rank-1 update in
+// LDS, against a row (S = 1) or a column (S = the row stride) that shares
+// no element with the written ones.  The restrict qualifiers let the
+// compiler issue an unrolled group's reads before its writes; without them
+// every iteration waits for the previous write.
+template <class F, int S>
+__device__ __forceinline__ void row_axpy(F *__restrict__ cr, const F *__restrict__ cp, F l, int c1, int c2) {
+#pragma unroll 8
+    for (int cc = c1; cc < c2; ++cc) cr[cc] = fma(-l, cp[cc * S], cr[cc]);
 }
 
 // ------------------------------------------------------------------- LU
@@ -233,7 +359,7 @@ __device__ int lu_panel(F *C, int m, int w, int *perm, int *piv, F *red_v, int *
             for (int r = c + 1 + t; r < m; r += NT) {
                 const F l = C[r * LDC + c] / pv;
                 C[r * LDC + c] = l;
-                for (int cc = c + 1; cc < w; ++cc) C[r * LDC + cc] = fma(-l, C[c * LDC + cc], C[r * LDC + cc]);
+                row_axpy<F, 1>(C + r * LDC, C + c * LDC, l, c + 1, w);
             }
         } else if (info == 0) {
             info = c + 1;
@@ -322,9 +448,16 @@ __device__ int lu_factor(F *G, int n, unsigned char *sm, F *gLinv, F *gUinv) {
         __syncthreads();
         const int pinfo = lu_panel<F, NB, NT>(C, m, w, perm + j0, piv + j0, red_v, red_r);
         if (info == 0 && pinfo != 0) info = j0 + pinfo;
-        for (int e = t; e < m * w; e += NT) {
-            const int r = e / w, c = e - r * w;
-            G[ix(perm[j0 + r], j0 + c, n)] = C[r * LDC + c];
+        if (w == NB) {
+            for (int e = t; e < m * NB; e += NT) {
+                const int r = e / NB, c = e % NB;
+                G[ix(perm[j0 + r], j0 + c, n)] = C[r * LDC + c];
+            }
+        } else {
+            for (int e = t; e < m * w; e += NT) {
+                const int r = e / w, c = e - r * w;
+                G[ix(perm[j0 + r], j0 + c, n)] = C[r * LDC + c];
+            }
         }
         lu_tri_inverses<F, NB, NT>(C, w, Li, Ui, gLinv + (int64_t)(j0 / NB) * NB * NB,
                                    gUinv + (int64_t)(j0 / NB) * NB * NB);
@@ -485,14 +618,20 @@ __device__ int chol_factor(F *G, int n, unsigned char *sm, F *gLinv) {
             __syncthreads();
             for (int r = c + 1 + t; r < m; r += NT) {
                 const F l = C[r * LDC + c];
-                const int lim = min(r, w - 1);
-                for (int cc = c + 1; cc <= lim; ++cc) C[r * LDC + cc] = fma(-l, C[cc * LDC + c], C[r * LDC + cc]);
+                row_axpy<F, LDC>(C + r * LDC, C + c, l, c + 1, min(r, w - 1) + 1);
             }
             __syncthreads();
         }
-        for (int e = t; e < m * w; e += NT) {
-            const int r = e / w, c = e - r * w;
-            if (c <= r) G[ix(j0 + r, j0 + c, n)] = C[r * LDC + c];
+        if (w == NB) {
+            for (int e = t; e < m * NB; e += NT) {
+                const int r = e / NB, c = e % NB;
+                if (c <= r) G[ix(j0 + r, j0 + c, n)] = C[r * LDC + c];
+            }
+        } else {
+            for (int e = t; e < m * w; e += NT) {
+                const int r = e / w, c = e - r * w;
+                if (c <= r) G[ix(j0 + r, j0 + c, n)] = C[r * LDC + c];
+            }
         }
         chol_tri_inverse<F, NB, NT>(C, w, Li, gLinv + (int64_t)(j0 / NB) * NB * NB);
         __syncthreads();
