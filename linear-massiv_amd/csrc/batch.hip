@@ -364,8 +364,12 @@ __device__ int lu_panel(F *C, int m, int w, int *perm, int *piv, F *red_v, int *
         } else if (info == 0) {
             info = c + 1;
         }
-        __syncthreads();
+        // No barrier here: the next column's search reads, in each thread,
+        // the rows this thread just updated (rows c + 1 + t + k NT both
+        // times); its interchange touches other threads' rows only after the
+        // search's barrier.
     }
+    __syncthreads();
     return info;
 }
 
@@ -608,13 +612,17 @@ __device__ int chol_factor(F *G, int n, unsigned char *sm, F *gLinv) {
         }
         __syncthreads();
         // column by column: sqrt of the pivot, true division below it, then
-        // the rank-1 update of the block column's remaining lower part
+        // the rank-1 update of the block column's remaining lower part.  The
+        // square roots wait in dg and enter the diagonal after the loop, so
+        // every thread's read of the pivot needs no barrier against its
+        // overwrite (the updates read only below the diagonal).
+        F *dg = red_v;
         for (int c = 0; c < w; ++c) {
             const F d = C[c * LDC + c];
             if (!(d > F(0))) return j0 + c + 1;
             const F sd = sqrt(d);
-            __syncthreads();
-            for (int r = c + t; r < m; r += NT) C[r * LDC + c] = (r == c) ? sd : C[r * LDC + c] / sd;
+            if (t == 0) dg[c] = sd;
+            for (int r = c + 1 + t; r < m; r += NT) C[r * LDC + c] = C[r * LDC + c] / sd;
             __syncthreads();
             for (int r = c + 1 + t; r < m; r += NT) {
                 const F l = C[r * LDC + c];
@@ -622,6 +630,8 @@ __device__ int chol_factor(F *G, int n, unsigned char *sm, F *gLinv) {
             }
             __syncthreads();
         }
+        for (int c = t; c < w; c += NT) C[c * LDC + c] = dg[c];
+        __syncthreads();
         if (w == NB) {
             for (int e = t; e < m * NB; e += NT) {
                 const int r = e / NB, c = e % NB;
@@ -636,7 +646,6 @@ __device__ int chol_factor(F *G, int n, unsigned char *sm, F *gLinv) {
         chol_tri_inverse<F, NB, NT>(C, w, Li, gLinv + (int64_t)(j0 / NB) * NB * NB);
         __syncthreads();
     }
-    (void)red_v;
     return 0;
 }
 
