@@ -413,27 +413,41 @@ __device__ int lu_factor(F *G, int n, unsigned char *sm, F *gLinv, F *gUinv) {
                         acc[q][ct][r4] = (rt < RT && row < n && col < w) ? G[ix(perm[row], j0 + col, n)] : F(0);
                     }
             }
+            // The operands of a k step are loaded unconditionally (addresses
+            // clamped into the matrix, out-of-range values zeroed after) and
+            // all before the MFMAs: loads under a per-tile branch wait for
+            // memory one tile after another.  The A rows' storage indices do
+            // not change over the k loop.
+            int arow[MRT];
+#pragma unroll
+            for (int q = 0; q < MRT; ++q) arow[q] = perm[min(j0 + (wave + q * NW) * 16 + lr, n - 1)];
             for (int k0 = 0; k0 < j0; k0 += 16) {
-                F bv[CT][4];
+                int brow[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) brow[s] = perm[k0 + lk * 4 + s];
+                F bv[CT][4], av[MRT][4];
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         const int col = ct * 16 + lr;
-                        bv[ct][s] = col < w ? G[ix(perm[k0 + lk * 4 + s], j0 + col, n)] : F(0);
+                        const F v = G[ix(brow[s], j0 + min(col, w - 1), n)];
+                        bv[ct][s] = col < w ? v : F(0);
                     }
+#pragma unroll
+                for (int q = 0; q < MRT; ++q)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) av[q][s] = G[ix(arow[q], k0 + lk * 4 + s, n)];
 #pragma unroll
                 for (int q = 0; q < MRT; ++q) {
                     const int rt = wave + q * NW;
                     if (rt < RT) {
-                        const int row = j0 + rt * 16 + lr;
-                        F av[4];
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) av[s] = row < n ? -G[ix(perm[row], k0 + lk * 4 + s, n)] : F(0);
+                        const bool live = j0 + rt * 16 + lr < n;
 #pragma unroll
                         for (int s = 0; s < 4; ++s)
 #pragma unroll
-                            for (int ct = 0; ct < CT; ++ct) acc[q][ct] = mfma(av[s], bv[ct][s], acc[q][ct]);
+                            for (int ct = 0; ct < CT; ++ct)
+                                acc[q][ct] = mfma(live ? -av[q][s] : F(0), bv[ct][s], acc[q][ct]);
                     }
                 }
             }
@@ -484,27 +498,38 @@ __device__ int lu_factor(F *G, int n, unsigned char *sm, F *gLinv, F *gUinv) {
                         acc[q][rt][r4] = (ct < NCT && col < n) ? G[ix(perm[row], col, n)] : F(0);
                     }
             }
+            // loads as in the block-column update: all of a k step's
+            // operands in flight before its MFMAs
+            int arow[CT];
+#pragma unroll
+            for (int rt = 0; rt < CT; ++rt) arow[rt] = perm[j0 + rt * 16 + lr];
             for (int k0 = 0; k0 < j0; k0 += 16) {
                 F av[CT][4];
 #pragma unroll
                 for (int rt = 0; rt < CT; ++rt)
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) av[rt][s] = -G[ix(perm[j0 + rt * 16 + lr], k0 + lk * 4 + s, n)];
+                    for (int s = 0; s < 4; ++s) av[rt][s] = -G[ix(arow[rt], k0 + lk * 4 + s, n)];
                 int srow[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) srow[s] = perm[k0 + lk * 4 + s];
+                F bv[MCT][4];
+#pragma unroll
+                for (int q = 0; q < MCT; ++q) {
+                    const int col = c0 + (wave + q * NW) * 16 + lr;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const F v = G[ix(srow[s], min(col, n - 1), n)];
+                        bv[q][s] = col < n ? v : F(0);
+                    }
+                }
 #pragma unroll
                 for (int q = 0; q < MCT; ++q) {
                     const int ct = wave + q * NW;
                     if (ct < NCT) {
-                        const int col = c0 + ct * 16 + lr;
-                        F bv[4];
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) bv[s] = col < n ? G[ix(srow[s], col, n)] : F(0);
 #pragma unroll
                         for (int s = 0; s < 4; ++s)
 #pragma unroll
-                            for (int rt = 0; rt < CT; ++rt) acc[q][rt] = mfma(av[rt][s], bv[s], acc[q][rt]);
+                            for (int rt = 0; rt < CT; ++rt) acc[q][rt] = mfma(av[rt][s], bv[q][s], acc[q][rt]);
                     }
                 }
             }
@@ -574,27 +599,34 @@ __device__ int chol_factor(F *G, int n, unsigned char *sm, F *gLinv) {
                         acc[q][ct][r4] = (rt < RT && rl < m && col < w && col <= rl) ? G[ix(j0 + rl, j0 + col, n)] : F(0);
                     }
             }
+            // all of a k step's operands in flight before its MFMAs (as in
+            // lu_factor)
             for (int k0 = 0; k0 < j0; k0 += 16) {
-                F bv[CT][4];
+                F bv[CT][4], av[MRT][4];
 #pragma unroll
                 for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         const int col = ct * 16 + lr;
-                        bv[ct][s] = col < w ? G[ix(j0 + col, k0 + lk * 4 + s, n)] : F(0);
+                        const F v = G[ix(j0 + min(col, w - 1), k0 + lk * 4 + s, n)];
+                        bv[ct][s] = col < w ? v : F(0);
                     }
+#pragma unroll
+                for (int q = 0; q < MRT; ++q) {
+                    const int row = min(j0 + (wave + q * NW) * 16 + lr, n - 1);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) av[q][s] = G[ix(row, k0 + lk * 4 + s, n)];
+                }
 #pragma unroll
                 for (int q = 0; q < MRT; ++q) {
                     const int rt = wave + q * NW;
                     if (rt < RT) {
-                        const int row = j0 + rt * 16 + lr;
-                        F av[4];
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) av[s] = row < n ? -G[ix(row, k0 + lk * 4 + s, n)] : F(0);
+                        const bool live = j0 + rt * 16 + lr < n;
 #pragma unroll
                         for (int s = 0; s < 4; ++s)
 #pragma unroll
-                            for (int ct = 0; ct < CT; ++ct) acc[q][ct] = mfma(av[s], bv[ct][s], acc[q][ct]);
+                            for (int ct = 0; ct < CT; ++ct)
+                                acc[q][ct] = mfma(live ? -av[q][s] : F(0), bv[ct][s], acc[q][ct]);
                     }
                 }
             }
