@@ -25,6 +25,40 @@ namespace {
 
 constexpr int DIAG = 64;
 
+// Rows [0, rows) of a 64-column block at G (row stride ldg) into S[r][c],
+// zeros below them: wave w loads rows 16w .. 16w+15, lane c column c; the
+// loads are unconditional at clamped addresses.
+template <class T>
+__device__ inline void stage_rows(T (*S)[DIAG + 1], const T *G, int64_t ldg, int rows, int cols, int w, int c) {
+    T v[16];
+    const T *Gc = G + min(c, cols - 1);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = Gc[(int64_t)min(16 * w + u, rows - 1) * ldg];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(v[u]));
+#pragma unroll
+    for (int u = 0; u < 16; ++u) S[16 * w + u][c] = (16 * w + u < rows && c < cols) ? v[u] : T(0);
+}
+
+// D += A B over a 64 x 64 x 64 product from LDS, a(r, k) = Ap[r ar + k ak],
+// b(k, c) = Bp[k bk + c bc]: wave wv takes rows 16wv .. 16wv+15 and the four
+// 16-column tiles, acc[ct] holding tile ct (MFMA 16 x 16 x 4; lane l gives
+// a(16wv + l % 16, k + l / 16) and b(k + l / 16, 16ct + l % 16)).
+template <class T>
+__device__ inline void tile_product(const T *Ap, int ar, int ak, const T *Bp, int bk, int bc, int wv, int lane,
+                                    typename V4<T>::type (&acc)[4]) {
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+    for (int k0 = 0; k0 < DIAG; k0 += 4) {
+        const T a = Ap[(16 * wv + lr) * ar + (k0 + lk) * ak];
+        T b[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) b[ct] = Bp[(k0 + lk) * bk + (16 * ct + lr) * bc];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma(a, b[ct], acc[ct]);
+    }
+}
+
 // In-place lower Cholesky of an nb x nb block (nb <= 64) by four waves:
 // lane i holds row i, wave w its columns 16w .. 16w+15.  Step j: the wave
 // that holds column j takes l_jj = sqrt(a_jj) from lane j (v_readlane),
@@ -44,12 +78,20 @@ constexpr int DIAG = 64;
 // it.  Every wave applies each finished row of X to all of its rows (zero
 // multipliers for the rows before it), with no branch.  X^T, upper, goes
 // to WT (row stride ldw), so that B L^{-T} is a plain product B WT.
-template <class T>
+//
+// The first half of a 128-column diagonal block (MODE 1) goes on with the
+// rows below it, k1 of them: L10 = A10 L^{-T} and A11 -= L10 L10^T (lower),
+// both in place.  The second half (MODE 2) goes on with the off-diagonal
+// block of the whole block's inverse: WT01 = -WT00 (L10^T WT11), with L10
+// the DIAG columns to the left of A and WT00 / WT01 the blocks DIAG rows up
+// in WT.  The products run on MFMA from LDS (tile_product).
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
                                                          int32_t *__restrict__ info, T *__restrict__ WT,
-                                                         int64_t ldw) {
+                                                         int64_t ldw, int k1) {
     __shared__ T S[DIAG][DIAG + 1];
-    __shared__ __attribute__((aligned(16))) T C[DIAG][DIAG];       // C[j][i] = l_ij
+    // C[j][i] = l_ij; afterwards the products' operand / result block
+    __shared__ __attribute__((aligned(16))) T C[DIAG][DIAG + 2];
     __shared__ __attribute__((aligned(16))) T R[DIAG][DIAG + 1];   // R[k][t] = x_kt
     __shared__ int sbad[4];
     // the wave index, uniform in a scalar register
@@ -155,6 +197,71 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
             if (sbad[q] != 0 && (first == 0 || sbad[q] < first)) first = sbad[q];
         if (first != 0 && *info == 0) *info = first;
     }
+    if constexpr (MODE == 0) return;
+    typedef typename V4<T>::type V;
+    const int lr = i & 15, lk = i >> 4;
+    __syncthreads();
+    if constexpr (MODE == 1) {
+        // A10 (k1 rows) into S
+        T *A10 = A + (int64_t)DIAG * lda, *A11 = A10 + DIAG;
+        stage_rows<T>(S, A10, lda, k1, DIAG, w, i);
+        __syncthreads();
+        // L10 = A10 X^T (X^T[k][c] = R[c][k]), to A10 and to C
+        V acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
+        tile_product<T>(&S[0][0], DIAG + 1, 1, &R[0][0], 1, DIAG + 1, w, i, acc);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int row = 16 * w + acc_row<T>(lk, r4), col = 16 * ct + lr;
+                C[row][col] = acc[ct][r4];
+                if (row < k1) A10[(int64_t)row * lda + col] = acc[ct][r4];
+            }
+        __syncthreads();
+        // A11 -= L10 L10^T on the lower triangle of its k1 x k1 block
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
+        tile_product<T>(&C[0][0], DIAG + 2, 1, &C[0][0], 1, DIAG + 2, w, i, acc);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int row = 16 * w + acc_row<T>(lk, r4), col = 16 * ct + lr;
+                T *p = A11 + (int64_t)min(row, k1 - 1) * lda + min(col, k1 - 1);
+                const T old = *p;
+                if (row < k1 && col <= row) *p = old - acc[ct][r4];
+            }
+    } else {
+        // L10 (nb rows, the DIAG columns left of A) into S
+        stage_rows<T>(S, A - DIAG, lda, nb, DIAG, w, i);
+        __syncthreads();
+        // T1 = L10^T WT11 (T1[k][c] = sum_r L10[r][k] X[c][r]) into C
+        V acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
+        tile_product<T>(&S[0][0], 1, DIAG + 1, &R[0][0], 1, DIAG + 1, w, i, acc);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) C[16 * w + acc_row<T>(lk, r4)][16 * ct + lr] = acc[ct][r4];
+        __syncthreads();
+        // WT00 into S; WT01 = -WT00 T1 for the nb columns
+        T *WT00 = WT - DIAG * ldw - DIAG, *WT01 = WT - DIAG * ldw;
+        stage_rows<T>(S, WT00, ldw, DIAG, DIAG, w, i);
+        __syncthreads();
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
+        tile_product<T>(&S[0][0], DIAG + 1, 1, &C[0][0], DIAG + 2, 1, w, i, acc);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                const int row = 16 * w + acc_row<T>(lk, r4), col = 16 * ct + lr;
+                if (col < nb) WT01[(int64_t)row * ldw + col] = -acc[ct][r4];
+            }
+    }
 }
 
 // C -= A B^T (lower tiles only with `lower`): FP64 / FP32 MFMA, or the BF16
@@ -192,11 +299,14 @@ void copy_block(hipStream_t s, const T *in, int64_t ldi, T *out, int64_t ldo, in
     LMX_LAUNCH_CHECK();
 }
 
-// Factors a 64 x 64 (or smaller) diagonal block and writes its L^{-T} to WT.
-template <class T>
-void diag_inv(hipStream_t s, int64_t nb, T *A, int64_t lda, int64_t col0, int32_t *d_info, T *WT, int64_t ldw) {
+// Factors a 64 x 64 (or smaller) diagonal block and writes its L^{-T} to WT
+// (MODE as potrf_diag_kernel).
+template <class T, int MODE = 0>
+void diag_inv(hipStream_t s, int64_t nb, T *A, int64_t lda, int64_t col0, int32_t *d_info, T *WT, int64_t ldw,
+              int64_t k1 = 0) {
     TimedRegion tr("potrf_diag", s);
-    hipLaunchKernelGGL(potrf_diag_kernel<T>, dim3(1), dim3(256), 0, s, (int)nb, A, lda, (int)col0, d_info, WT, ldw);
+    hipLaunchKernelGGL((potrf_diag_kernel<T, MODE>), dim3(1), dim3(256), 0, s, (int)nb, A, lda, (int)col0, d_info,
+                       WT, ldw, (int)k1);
     LMX_LAUNCH_CHECK();
 }
 
@@ -218,7 +328,15 @@ template <class T>
 void diag_block(hipStream_t s, bool bf16, int64_t kb, T *A11, int64_t lda, int64_t col0, int32_t *d_info, T *WT,
                 T *tmp) {
     if (kb <= DIAG) {
-        diag_inv(s, kb, A11, lda, col0, d_info, WT, BLK);
+        diag_inv<T>(s, kb, A11, lda, col0, d_info, WT, BLK);
+        return;
+    }
+    if (kb <= 2 * DIAG && !bf16) {
+        // two kernels: the first half with L10 and the second half's update,
+        // the second half with the inverse's off-diagonal block
+        const int64_t k1 = kb - DIAG;
+        diag_inv<T, 1>(s, DIAG, A11, lda, col0, d_info, WT, BLK, k1);
+        diag_inv<T, 2>(s, k1, A11 + DIAG * lda + DIAG, lda, col0 + DIAG, d_info, WT + DIAG * BLK + DIAG, BLK);
         return;
     }
     int64_t h = DIAG;
