@@ -61,7 +61,9 @@ extern "C" {
 /* Select and initialise a device (idempotent).  Replaces nothing in the
  * reference, which has no device. */
 int lmx_init(int device);
-/* Release device resources held by the library. */
+/* Release device resources held by the library: its streams, its memory
+ * pool and the freed scratch blocks it keeps per stream for reuse (up to
+ * 8 GiB per device). */
 int lmx_finalize(void);
 /* Thread-local description of the last error (never NULL). */
 const char *lmx_last_error(void);
