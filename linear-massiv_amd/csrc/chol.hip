@@ -26,16 +26,18 @@ namespace {
 constexpr int DIAG = 64;
 
 // Rows [0, rows) of a 64-column block at G (row stride ldg) into S[r][c],
-// zeros below them: wave w loads rows 16w .. 16w+15, lane c column c; the
-// loads are unconditional at clamped addresses.
+// zeros below them, in two parts: load_rows issues the loads (wave w rows
+// 16w .. 16w+15, lane c column c, unconditional at clamped addresses) and
+// put_rows stores them, so that other loads can be in flight in between.
 template <class T>
-__device__ inline void stage_rows(T (*S)[DIAG + 1], const T *G, int64_t ldg, int rows, int cols, int w, int c) {
-    T v[16];
+__device__ inline void load_rows(T (&v)[16], const T *G, int64_t ldg, int rows, int cols, int w, int c) {
     const T *Gc = G + min(c, cols - 1);
 #pragma unroll
     for (int u = 0; u < 16; ++u) v[u] = Gc[(int64_t)min(16 * w + u, rows - 1) * ldg];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(v[u]));
+}
+
+template <class T>
+__device__ inline void put_rows(T (*S)[DIAG + 1], const T (&v)[16], int rows, int cols, int w, int c) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) S[16 * w + u][c] = (16 * w + u < rows && c < cols) ? v[u] : T(0);
 }
@@ -59,25 +61,53 @@ __device__ inline void tile_product(const T *Ap, int ar, int ak, const T *Bp, in
     }
 }
 
+// Rows 16q .. 16q+15 of X = L^{-1} in wave q (lane i holds column i, x[u]
+// row 16q + u), once rows 0 .. 16q-1 of X are in R and columns 0 .. 16q+15
+// of L in C (C[j][r] = l_rj): x_r = (e_r - sum_{k<r} l_rk x_k) / l_rr, the
+// terms in ascending k, as the 64 x 64 inverse kernel's forward
+// substitution.  inverse_terms applies the rows before the group,
+// inverse_finish the group's own and writes each finished row to R.
+template <class T>
+__device__ inline void inverse_terms(const T (*C)[DIAG + 2], const T (*R)[DIAG + 1], T (&x)[16], int q, int i) {
+    for (int k = 0; k < 16 * q; ++k) {
+        const T xk = R[k][i];
+        T ck[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) ck[u] = C[k][16 * q + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] = fma(-ck[u], xk, x[u]);
+    }
+}
+
+template <class T>
+__device__ inline void inverse_finish(const T (*C)[DIAG + 2], T (*R)[DIAG + 1], T (&x)[16], int q, int i) {
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+        const int k = 16 * q + kk;
+        x[kk] = x[kk] / C[k][k];
+        R[k][i] = x[kk];
+        T ck[16];
+#pragma unroll
+        for (int u = kk + 1; u < 16; ++u) ck[u] = C[k][16 * q + u];
+#pragma unroll
+        for (int u = kk + 1; u < 16; ++u) x[u] = fma(-ck[u], x[kk], x[u]);
+    }
+}
+
 // In-place lower Cholesky of an nb x nb block (nb <= 64) by four waves:
-// lane i holds row i, wave w its columns 16w .. 16w+15.  Step j: the wave
-// that holds column j takes l_jj = sqrt(a_jj) from lane j (v_readlane),
-// divides its column by it (l_ij = a_ij / l_jj, a division, as
-// rawCholColumnSIMD), publishes the column in LDS and updates its own
-// columns after j; after one barrier the waves holding later columns apply
-// a_ic -= l_ij l_cj to all of theirs.  Every element takes its terms in
-// ascending j.  Lanes also update their row's columns above the diagonal;
-// those values are never stored.
+// lane i holds row i, wave w its columns 16w .. 16w+15.  Column j: the wave
+// that holds it takes l_jj = sqrt(a_jj) from lane j (v_readlane), divides
+// its column by it (l_ij = a_ij / l_jj, a division, as rawCholColumnSIMD),
+// publishes the column in LDS and updates its own columns after j, all 16
+// of its columns without a barrier; after one barrier the waves holding
+// later columns apply the 16 columns to all of theirs (a_ic -= l_ij l_cj).
+// Every element takes its terms in ascending j.  Lanes also update their
+// row's columns above the diagonal; those values are never stored.
 //
-// The same loop builds the inverse X = L^{-1}, one step behind: row k is
-// final at step k + 1 (x_k = (e_k - sum_{m<k} l_km x_m) / l_kk, the terms
-// in ascending m -- the order of the 64 x 64 inverse kernel's forward
-// substitution, with the identity beyond nb).  Lane i holds column i of X;
-// the rows 16g .. 16g+15 sit in wave (g + 1) mod 4, so that the division
-// that finishes row k runs beside the next column's square root, not after
-// it.  Every wave applies each finished row of X to all of its rows (zero
-// multipliers for the rows before it), with no branch.  X^T, upper, goes
-// to WT (row stride ldw), so that B L^{-T} is a plain product B WT.
+// The inverse X = L^{-1} (the identity beyond nb) is built beside it: lane
+// i holds column i of X, wave q rows 16q .. 16q+15 (inverse_rows), which it
+// finishes while the next wave factors its columns.  X^T, upper, goes to WT
+// (row stride ldw), so that B L^{-T} is a plain product B WT.
 //
 // The first half of a 128-column diagonal block (MODE 1) goes on with the
 // rows below it, k1 of them: L10 = A10 L^{-T} and A11 -= L10 L10^T (lower),
@@ -96,17 +126,41 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
     __shared__ int sbad[4];
     // the wave index, uniform in a scalar register
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), i = threadIdx.x & 63;
-    const int g = (w + 3) & 3;  // the group of rows of X this wave holds
+    const int lr = i & 15, lk = i >> 4;
     // coalesced loads through LDS (lane i reads column i of the wave's 16
     // rows), unconditional at addresses clamped into the block: a load under
-    // a per-lane condition waits for memory before the next one issues
+    // a per-lane condition waits for memory before the next one issues.  The
+    // products' operands after the factorisation are loaded here too (they
+    // do not depend on it), so all of the kernel's loads wait once:
+    //   MODE 1: p0 = A10's rows, p1 = A11's old values in the product's
+    //           accumulator layout;
+    //   MODE 2: p0 = L10's rows, p1 = WT00's rows.
+    T p0[16], p1[16];
     {
         const T *Ai = A + min(i, nb - 1);
         T v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) v[u] = Ai[(int64_t)min(16 * w + u, nb - 1) * lda];
+        if constexpr (MODE == 1) {
+            const T *A10 = A + (int64_t)DIAG * lda, *A11 = A10 + DIAG;
+            load_rows<T>(p0, A10, lda, k1, DIAG, w, i);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int row = 16 * w + acc_row<T>(lk, r4), col = 16 * ct + lr;
+                    p1[4 * ct + r4] = A11[(int64_t)min(row, k1 - 1) * lda + min(col, k1 - 1)];
+                }
+        } else if constexpr (MODE == 2) {
+            load_rows<T>(p0, A - DIAG, lda, nb, DIAG, w, i);
+            load_rows<T>(p1, WT - DIAG * ldw - DIAG, ldw, DIAG, DIAG, w, i);
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(v[u]));
+        if constexpr (MODE != 0) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) asm volatile("" : "+v"(p0[u]), "+v"(p1[u]));
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int r = 16 * w + u;
@@ -118,43 +172,22 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         a[u] = S[i][16 * w + u];
-        x[u] = 16 * g + u == i ? T(1) : T(0);
+        x[u] = 16 * w + u == i ? T(1) : T(0);
     }
-    // The steps run in four groups of 16, each unrolled, so that every
-    // register index is a compile-time constant.  All 64 steps run; rows
-    // and columns from nb on start at zero, may turn to NaN, and are never
-    // stored.  A published column is the identity's beyond nb (no real
-    // entry depends on those), which gives the inverse its identity padding.
+    // Four phases and four barriers.  Phase p: the waves from p on apply
+    // block p - 1's 16 columns to theirs (a rank-16 update, the terms in
+    // ascending j); wave p then factors its 16 columns with no barrier (it
+    // publishes each column in LDS and updates its own later columns);
+    // wave p - 1, idle from then on, finishes rows 16(p-1) .. 16p-1 of X.
+    // All 64 steps run; rows and columns from nb on start at zero, may turn
+    // to NaN, and are never stored.  A published column is the identity's
+    // beyond nb (no real entry depends on those), which gives the inverse
+    // its identity padding.
     int bad = 0;
-    for (int jo = 0; jo < 4; ++jo) {
-#pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-            const int j = 16 * jo + jj;
-            if (w == jo) {
-                const T d = lane_value(a[jj], j);
-                const T sjj = sqrt(d);
-                if (j < nb && !(d > T(0)) && bad == 0) bad = col0 + j + 1;
-                a[jj] = i == j ? sjj : a[jj] / sjj;
-                const T li = (i >= j && i < nb && j < nb) ? a[jj] : (i == j ? T(1) : T(0));
-                C[j][i] = li;
-                // this wave's columns after j (its own LDS writes are
-                // visible to its reads without a barrier)
-                T cj[16];
-#pragma unroll
-                for (int u = jj + 1; u < 16; ++u) cj[u] = C[j][16 * jo + u];
-#pragma unroll
-                for (int u = jj + 1; u < 16; ++u) a[u] = fma(-li, cj[u], a[u]);
-            }
-            // row k of X, one step behind
-            const int k = j - 1, ko = jj == 0 ? jo - 1 : jo, kk = (jj + 15) & 15;
-            if (j > 0 && w == ((ko + 1) & 3)) {
-                x[kk] = x[kk] / C[k][k];
-                R[k][i] = x[kk];
-            }
-            __syncthreads();
-            // the later groups: every column (row of X) of the wave, reads
-            // issued together
-            if (w > jo) {
+    for (int p = 0; p < 4; ++p) {
+        if (p > 0 && w >= p) {
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * (p - 1) + jj;
                 const T li = C[j][i];
                 T cj[16];
 #pragma unroll
@@ -162,33 +195,76 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
 #pragma unroll
                 for (int u = 0; u < 16; ++u) a[u] = fma(-li, cj[u], a[u]);
             }
-            // every row of X after k; a published column is zero above its
-            // diagonal, so the rows before k are left as they are (the
-            // diagonal term itself is masked)
-            if (j > 0) {
-                const T xk = R[k][i];
-                T ck[16];
+        }
+        if (w == p) {
+            // Column j's multiplier for column j + 1 comes by v_readlane,
+            // and its terms for the columns after j + 1 are applied one
+            // column later, so that no LDS round trip lies between one
+            // column's square root and the next one's.
+            T lp = T(0), cp[16];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) ck[u] = C[k][16 * g + u];
-                ck[kk] = g == ko ? T(0) : ck[kk];
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * p + jj;
+                const T d = lane_value(a[jj], j);
+                const T sjj = sqrt(d);
+                if (j < nb && !(d > T(0)) && bad == 0) bad = col0 + j + 1;
+                // column j - 1's terms, deferred
 #pragma unroll
-                for (int u = 0; u < 16; ++u) x[u] = fma(-ck[u], xk, x[u]);
+                for (int u = jj + 1; u < 16; ++u)
+                    if (jj > 0) a[u] = fma(-lp, cp[u], a[u]);
+                a[jj] = i == j ? sjj : a[jj] / sjj;
+                const T li = (i >= j && i < nb && j < nb) ? a[jj] : (i == j ? T(1) : T(0));
+                C[j][i] = li;
+                S[i][j] = a[jj];  // final: L's store goes through S
+                // (this wave's own LDS writes are visible to its reads
+                // without a barrier)
+#pragma unroll
+                for (int u = jj + 2; u < 16; ++u) cp[u] = C[j][16 * p + u];
+                if (jj < 15) a[jj + 1] = fma(-li, lane_value(li, j + 1), a[jj + 1]);
+                lp = li;
             }
         }
+        if (p > 0 && w == p - 1) {
+            inverse_terms<T>(C, R, x, w, i);
+            inverse_finish<T>(C, R, x, w, i);
+        }
+        __syncthreads();
     }
-    if (w == 0) {  // row 63 of X (group 3)
-        x[15] = x[15] / C[DIAG - 1][DIAG - 1];
-        R[DIAG - 1][i] = x[15];
-    }
-    if (i == 0) sbad[w] = bad;
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 16; ++u) S[i][16 * w + u] = a[u];
-    __syncthreads();
+    // L is final in S: its stores go now, beside the last rows of X.
 #pragma unroll 4
     for (int u = 0; u < 16; ++u) {
         const int r = 16 * w + u;
         if (r < nb && i <= r) A[(int64_t)r * lda + i] = S[r][i];
+    }
+    // Rows 48 .. 63 of X: the terms of rows 0 .. 47, four rows a wave (in
+    // ascending k, as inverse_terms), then the group's own in wave 3.
+    {
+        T y[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) y[t] = 48 + 4 * w + t == i ? T(1) : T(0);
+#pragma unroll 4
+        for (int k = 0; k < 48; ++k) {
+            const T xk = R[k][i];
+            T ck[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) ck[t] = C[k][48 + 4 * w + t];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) y[t] = fma(-ck[t], xk, y[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) R[48 + 4 * w + t][i] = y[t];
+    }
+    __syncthreads();
+    if (w == 3) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] = R[48 + u][i];
+        inverse_finish<T>(C, R, x, 3, i);
+    }
+    if (i == 0) sbad[w] = bad;
+    __syncthreads();
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+        const int r = 16 * w + u;
         WT[(int64_t)r * ldw + i] = R[i][r];  // WT[r][i] = x_ir
     }
     if (threadIdx.x == 0) {
@@ -199,12 +275,11 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
     }
     if constexpr (MODE == 0) return;
     typedef typename V4<T>::type V;
-    const int lr = i & 15, lk = i >> 4;
     __syncthreads();
     if constexpr (MODE == 1) {
         // A10 (k1 rows) into S
         T *A10 = A + (int64_t)DIAG * lda, *A11 = A10 + DIAG;
-        stage_rows<T>(S, A10, lda, k1, DIAG, w, i);
+        put_rows<T>(S, p0, k1, DIAG, w, i);
         __syncthreads();
         // L10 = A10 X^T (X^T[k][c] = R[c][k]), to A10 and to C
         V acc[4];
@@ -229,13 +304,11 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
 #pragma unroll
             for (int r4 = 0; r4 < 4; ++r4) {
                 const int row = 16 * w + acc_row<T>(lk, r4), col = 16 * ct + lr;
-                T *p = A11 + (int64_t)min(row, k1 - 1) * lda + min(col, k1 - 1);
-                const T old = *p;
-                if (row < k1 && col <= row) *p = old - acc[ct][r4];
+                if (row < k1 && col <= row) A11[(int64_t)row * lda + col] = p1[4 * ct + r4] - acc[ct][r4];
             }
     } else {
         // L10 (nb rows, the DIAG columns left of A) into S
-        stage_rows<T>(S, A - DIAG, lda, nb, DIAG, w, i);
+        put_rows<T>(S, p0, nb, DIAG, w, i);
         __syncthreads();
         // T1 = L10^T WT11 (T1[k][c] = sum_r L10[r][k] X[c][r]) into C
         V acc[4];
@@ -248,8 +321,8 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
             for (int r4 = 0; r4 < 4; ++r4) C[16 * w + acc_row<T>(lk, r4)][16 * ct + lr] = acc[ct][r4];
         __syncthreads();
         // WT00 into S; WT01 = -WT00 T1 for the nb columns
-        T *WT00 = WT - DIAG * ldw - DIAG, *WT01 = WT - DIAG * ldw;
-        stage_rows<T>(S, WT00, ldw, DIAG, DIAG, w, i);
+        T *WT01 = WT - DIAG * ldw;
+        put_rows<T>(S, p1, DIAG, DIAG, w, i);
         __syncthreads();
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
