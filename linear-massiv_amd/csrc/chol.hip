@@ -461,7 +461,10 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
         }
         if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
         const int64_t narrow = std::min(nb, rest);
-        if (rest > 0) {
+        if (rest > 0 && !bf16) {
+            // the next block column, on and below its diagonal, in one launch
+            syrk_trap<T>(s, rest, narrow, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
+        } else if (rest > 0) {
             chol_update<T>(s, bf16, true, narrow, narrow, kb, A21, lda, A21, A21 + kb);
             if (rest > narrow)
                 chol_update<T>(s, bf16, false, rest - narrow, narrow, kb, A21 + narrow * lda, lda, A21,

@@ -64,12 +64,24 @@ __device__ inline f4 mfma16x16x4(float a, float b, f4 c) {
 // of ids, and the grouped order inside a range keeps GROUP_M tile-rows
 // sharing B panels in that XCD's L2.
 // With `lower` the grid enumerates only the tiles on or below the diagonal
-// of a square tile grid (tn <= tm), row by row.
+// of a lower-trapezoidal tile grid (tn <= tm, tiles_n <= tiles_m): the
+// triangle of the first tiles_n tile rows row by row, then the full rows
+// below it.
+__host__ __device__ inline int lower_tiles(int tiles_m, int tiles_n) {
+    return tiles_n * (tiles_n + 1) / 2 + (tiles_m - tiles_n) * tiles_n;
+}
+
 __device__ inline void tile_of(int tiles_m, int tiles_n, bool lower, int &tm, int &tn) {
-    int nwg = lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
+    int nwg = lower ? lower_tiles(tiles_m, tiles_n) : tiles_m * tiles_n;
     int bid = blockIdx.x;
     if ((nwg & 7) == 0) bid = (bid & 7) * (nwg >> 3) + (bid >> 3);
     if (lower) {
+        const int tri = tiles_n * (tiles_n + 1) / 2;
+        if (bid >= tri) {
+            tm = tiles_n + (bid - tri) / tiles_n;
+            tn = (bid - tri) % tiles_n;
+            return;
+        }
         int r = (int)((sqrtf(8.0f * (float)bid + 1.0f) - 1.0f) * 0.5f);
         while (r * (r + 1) / 2 > bid) --r;
         while ((r + 1) * (r + 2) / 2 <= bid) ++r;
@@ -263,8 +275,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
 //   distinct banks.
 //   With TB, B is stored transposed (n x k, so op(B) = B^T, as in A A^T):
 //   its k runs along its rows too, and its tiles load and store like A's.
-//   With `lower`, C is square and only the tiles on or below its diagonal
-//   are launched, and only elements on or below it are written (SYRK).
+//   With `lower`, C is lower-trapezoidal (square for a SYRK) and only the
+//   tiles on or below its diagonal are launched, and only elements on or
+//   below it are written.
 template <class T, int BM, int BN, bool TB>
 __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alpha, const T *__restrict__ A,
                                                       int64_t lda, const T *__restrict__ B, int64_t ldb, T beta,
@@ -610,7 +623,7 @@ template <class S, bool SPLIT, int BM, int BN, int VE>
 void launch_bf16_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, float alpha, const S *A, int64_t lda,
                       const S *B, int64_t ldb, float beta, float *C, int64_t ldc, int lower, size_t lds_footprint) {
     const int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
-    const dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n);
+    const dim3 grid(lower ? lower_tiles(tiles_m, tiles_n) : tiles_m * tiles_n);
     const size_t st = 2 * (size_t)(SPLIT ? 2 : 1) * (BM + BN) * BF_SK * sizeof(unsigned short);
     const size_t reserve = lds_footprint > st ? lds_footprint - st : 0;
 #define LMX_BF16_CASE(TA_, TB_)                                                                                   \
@@ -699,7 +712,7 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
                  int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0,
                  int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}, size_t lds_footprint = 0) {
     int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
-    dim3 grid(lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n, (unsigned)batch), block(256);
+    dim3 grid(lower ? lower_tiles(tiles_m, tiles_n) : tiles_m * tiles_n, (unsigned)batch), block(256);
     // lds_footprint: the LDS one workgroup should hold in all (its tiles plus
     // an unused reserve), so that fewer workgroups share a CU; 0 = no reserve.
     // A GEMM that shares its CUs with FP32 panel workgroups keeps the first
@@ -835,8 +848,13 @@ void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t
     if (n <= 0) return;
     if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
     TimedRegion tr(sizeof(T) == 8 ? "syrk" : "ssyrk", s, (double)n * (double)(n + 1) * (double)k);
+    // FP64 takes 64 x 64 tiles at every size: the 128 x 128 FP64 tile holds
+    // one wave a SIMD and ran at 20.8 TF alone (n = 7936, k = 256) against
+    // 29.4 TF (40.5 against 28.8 at k = 512).  FP32 keeps 128 x 128 tiles for
+    // large outputs: with 64 x 64 its SYRK was faster but took the CUs from
+    // the factorisation's critical path (mixed Cholesky 34.5 against 33.5 ms).
     int64_t big_tiles = cdiv(n, 128) * (cdiv(n, 128) + 1) / 2;
-    if (big_tiles >= 512)
+    if (sizeof(T) == 4 && big_tiles >= 512)
         launch_tile<T, 128, 128, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
                                      ldc, 1);
     else
@@ -877,6 +895,24 @@ void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A
                 float *C, int64_t ldc) {
     syrk_impl<float>(s, n, k, alpha, A, lda, beta, C, ldc);
 }
+
+// C (m x n, m >= n) = alpha A A[0:n]^T + beta C on the elements with row >=
+// column: the update of a block column below and on the diagonal, in one
+// launch of 64 x 64 tiles.
+template <class T>
+void syrk_trap(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
+               int64_t ldc) {
+    if (m <= 0 || n <= 0) return;
+    if (n > m) fail_arg("syrk_trap: more columns than rows");
+    if (m > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk_trap: dimension too large");
+    const double elems = (double)n * (double)(n + 1) / 2 + (double)(m - n) * (double)n;
+    TimedRegion tr(sizeof(T) == 8 ? "gemm64" : "sgemm64", s, 2.0 * elems * (double)k);
+    launch_tile<T, 64, 64, 16>(s, false, true, (int)m, (int)n, (int)k, alpha, A, lda, A, lda, beta, C, ldc, 1);
+}
+template void syrk_trap<double>(hipStream_t, int64_t, int64_t, int64_t, double, const double *, int64_t, double,
+                                double *, int64_t);
+template void syrk_trap<float>(hipStream_t, int64_t, int64_t, int64_t, float, const float *, int64_t, float, float *,
+                               int64_t);
 
 void gemm(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
           const double *A, int64_t lda, const double *B, int64_t ldb, double beta, double *C,

@@ -57,6 +57,11 @@ void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double 
                 double beta, double *C, int64_t ldc);
 void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A, int64_t lda, float beta,
                 float *C, int64_t ldc);
+// C (m x n, m >= n) = alpha A A[0:n]^T + beta C on the elements with row >=
+// column (the lower trapezoid), one launch.
+template <class T>
+void syrk_trap(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
+               int64_t ldc);
 
 // Lower triangle of C = alpha op(A) op(A)^T + beta C: op(A) = A (n x k) or,
 // with `trans`, A^T (A stored k x n, so C = A^T A).
