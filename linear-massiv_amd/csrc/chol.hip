@@ -25,8 +25,8 @@ namespace {
 
 constexpr int DIAG = 64;
 
-// Rows [0, rows) of a 64-column block at G (row stride ldg) into S[r][c],
-// zeros below them, in two parts: load_rows issues the loads (wave w rows
+// Rows [0, rows) of a 64-column block at G (row stride ldg) into S[r][c]
+// (an LDS array of row stride LD), zeros below them, in two parts: load_rows issues the loads (wave w rows
 // 16w .. 16w+15, lane c column c, unconditional at clamped addresses) and
 // put_rows stores them, so that other loads can be in flight in between.
 template <class T>
@@ -36,8 +36,8 @@ __device__ inline void load_rows(T (&v)[16], const T *G, int64_t ldg, int rows, 
     for (int u = 0; u < 16; ++u) v[u] = Gc[(int64_t)min(16 * w + u, rows - 1) * ldg];
 }
 
-template <class T>
-__device__ inline void put_rows(T (*S)[DIAG + 1], const T (&v)[16], int rows, int cols, int w, int c) {
+template <class T, int LD>
+__device__ inline void put_rows(T (*S)[LD], const T (&v)[16], int rows, int cols, int w, int c) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) S[16 * w + u][c] = (16 * w + u < rows && c < cols) ? v[u] : T(0);
 }
@@ -116,13 +116,15 @@ __device__ inline void inverse_finish(const T (*C)[DIAG + 2], T (*R)[DIAG + 1], 
 // the DIAG columns to the left of A and WT00 / WT01 the blocks DIAG rows up
 // in WT.  The products run on MFMA from LDS (tile_product).
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
+__global__ __launch_bounds__(256, 2) void potrf_diag_kernel(int nb, T *__restrict__ A, int64_t lda, int col0,
                                                          int32_t *__restrict__ info, T *__restrict__ WT,
                                                          int64_t ldw, int k1) {
-    __shared__ T S[DIAG][DIAG + 1];
     // C[j][i] = l_ij; afterwards the products' operand / result block
     __shared__ __attribute__((aligned(16))) T C[DIAG][DIAG + 2];
-    __shared__ __attribute__((aligned(16))) T R[DIAG][DIAG + 1];   // R[k][t] = x_kt
+    // R[k][t] = x_kt; before that the block as loaded, afterwards a
+    // product's operand (no third array: with 67 KB of LDS the kernel shares
+    // a CU with two workgroups of the lookahead SYRK)
+    __shared__ __attribute__((aligned(16))) T R[DIAG][DIAG + 1];
     __shared__ int sbad[4];
     // the wave index, uniform in a scalar register
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), i = threadIdx.x & 63;
@@ -164,14 +166,14 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int r = 16 * w + u;
-            S[r][i] = (r < nb && i <= r) ? v[u] : T(0);
+            R[r][i] = (r < nb && i <= r) ? v[u] : T(0);  // R stages the block
         }
     }
     __syncthreads();
     T a[16], x[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        a[u] = S[i][16 * w + u];
+        a[u] = R[i][16 * w + u];
         x[u] = 16 * w + u == i ? T(1) : T(0);
     }
     // Four phases and four barriers.  Phase p: the waves from p on apply
@@ -215,7 +217,6 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
                 a[jj] = i == j ? sjj : a[jj] / sjj;
                 const T li = (i >= j && i < nb && j < nb) ? a[jj] : (i == j ? T(1) : T(0));
                 C[j][i] = li;
-                S[i][j] = a[jj];  // final: L's store goes through S
                 // (this wave's own LDS writes are visible to its reads
                 // without a barrier)
 #pragma unroll
@@ -230,11 +231,12 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
         }
         __syncthreads();
     }
-    // L is final in S: its stores go now, beside the last rows of X.
+    // L is final in C (C[c][r] = l_rc on and below the diagonal): its
+    // stores go now, beside the last rows of X.
 #pragma unroll 4
     for (int u = 0; u < 16; ++u) {
         const int r = 16 * w + u;
-        if (r < nb && i <= r) A[(int64_t)r * lda + i] = S[r][i];
+        if (r < nb && i <= r) A[(int64_t)r * lda + i] = C[i][r];
     }
     // Rows 48 .. 63 of X: the terms of rows 0 .. 47, four rows a wave (in
     // ascending k, as inverse_terms), then the group's own in wave 3.
@@ -277,15 +279,17 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
     typedef typename V4<T>::type V;
     __syncthreads();
     if constexpr (MODE == 1) {
-        // A10 (k1 rows) into S
+        // A10 (k1 rows) into C
         T *A10 = A + (int64_t)DIAG * lda, *A11 = A10 + DIAG;
-        put_rows<T>(S, p0, k1, DIAG, w, i);
+        put_rows<T, DIAG + 2>(C, p0, k1, DIAG, w, i);
         __syncthreads();
-        // L10 = A10 X^T (X^T[k][c] = R[c][k]), to A10 and to C
+        // L10 = A10 X^T (X^T[k][c] = R[c][k]), to A10 and, once every wave
+        // has read A10, to C
         V acc[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
-        tile_product<T>(&S[0][0], DIAG + 1, 1, &R[0][0], 1, DIAG + 1, w, i, acc);
+        tile_product<T>(&C[0][0], DIAG + 2, 1, &R[0][0], 1, DIAG + 1, w, i, acc);
+        __syncthreads();
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
@@ -307,26 +311,27 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int nb, T *__restrict__
                 if (row < k1 && col <= row) A11[(int64_t)row * lda + col] = p1[4 * ct + r4] - acc[ct][r4];
             }
     } else {
-        // L10 (nb rows, the DIAG columns left of A) into S
-        put_rows<T>(S, p0, nb, DIAG, w, i);
+        // L10 (nb rows, the DIAG columns left of A) into C
+        put_rows<T, DIAG + 2>(C, p0, nb, DIAG, w, i);
         __syncthreads();
-        // T1 = L10^T WT11 (T1[k][c] = sum_r L10[r][k] X[c][r]) into C
+        // T1 = L10^T WT11 (T1[k][c] = sum_r L10[r][k] X[c][r]) into R, once
+        // every wave has read X
         V acc[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
-        tile_product<T>(&S[0][0], 1, DIAG + 1, &R[0][0], 1, DIAG + 1, w, i, acc);
+        tile_product<T>(&C[0][0], 1, DIAG + 2, &R[0][0], 1, DIAG + 1, w, i, acc);
+        __syncthreads();
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4) C[16 * w + acc_row<T>(lk, r4)][16 * ct + lr] = acc[ct][r4];
-        __syncthreads();
-        // WT00 into S; WT01 = -WT00 T1 for the nb columns
+            for (int r4 = 0; r4 < 4; ++r4) R[16 * w + acc_row<T>(lk, r4)][16 * ct + lr] = acc[ct][r4];
+        // WT00 into C; WT01 = -WT00 T1 for the nb columns
         T *WT01 = WT - DIAG * ldw;
-        put_rows<T>(S, p1, DIAG, DIAG, w, i);
+        put_rows<T, DIAG + 2>(C, p1, DIAG, DIAG, w, i);
         __syncthreads();
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] = V{};
-        tile_product<T>(&S[0][0], DIAG + 1, 1, &C[0][0], DIAG + 2, 1, w, i, acc);
+        tile_product<T>(&C[0][0], DIAG + 2, 1, &R[0][0], DIAG + 1, 1, w, i, acc);
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
