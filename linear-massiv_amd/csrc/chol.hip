@@ -477,9 +477,19 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
         }
         LMX_HIP(hipEventRecord(la.ready, s));
         LMX_HIP(hipStreamWaitEvent(la.aux, la.ready, 0));
-        if (rest > narrow)
-            chol_update<T>(la.aux, bf16, true, rest - narrow, rest - narrow, kb, A21 + narrow * lda, lda,
-                           A21 + narrow * lda, A21 + narrow * lda + kb + narrow);
+        if (rest > narrow) {
+            if constexpr (sizeof(T) == 8) {
+                // At most two SYRK workgroups a CU (80 KB of LDS each): when
+                // one ends, the diagonal kernel (67 KB, 4 waves) fits in its
+                // place.  With three a CU it waited for the SYRK to drain,
+                // up to 300 us a block (n = 8192: 12.7 against 13.1 ms).
+                syrk_lower_sharing(la.aux, rest - narrow, kb, -1.0, A21 + narrow * lda, lda, 1.0,
+                                   A21 + narrow * lda + kb + narrow, lda, (size_t)80 * 1024);
+            } else {
+                chol_update<T>(la.aux, bf16, true, rest - narrow, rest - narrow, kb, A21 + narrow * lda, lda,
+                               A21 + narrow * lda, A21 + narrow * lda + kb + narrow);
+            }
+        }
         LMX_HIP(hipEventRecord(la.done, la.aux));
         pending = true;
     }

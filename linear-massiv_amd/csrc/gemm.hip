@@ -844,7 +844,7 @@ void gemm_batched_impl(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, 
 // Cholesky update) or, with `trans`, A^T (A^T A of matMulAtAP).
 template <class T>
 void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
-               int64_t ldc, bool trans = false) {
+               int64_t ldc, bool trans = false, size_t lds_footprint = 0) {
     if (n <= 0) return;
     if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
     TimedRegion tr(sizeof(T) == 8 ? "syrk" : "ssyrk", s, (double)n * (double)(n + 1) * (double)k);
@@ -859,7 +859,7 @@ void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t
                                      ldc, 1);
     else
         launch_tile<T, 64, 64, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
-                                   ldc, 1);
+                                   ldc, 1, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
 }
 
 }  // namespace
@@ -884,6 +884,11 @@ void gemm_bf16_rounded(hipStream_t s, bool ta, bool tb, int64_t m, int64_t n, in
 void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda, double beta,
                 double *C, int64_t ldc) {
     syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+void syrk_lower_sharing(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                        double beta, double *C, int64_t ldc, size_t lds_footprint) {
+    syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc, false, lds_footprint);
 }
 
 void syrk_lower_op(hipStream_t s, bool trans, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
