@@ -848,18 +848,12 @@ void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t
     if (n <= 0) return;
     if (n > INT32_MAX / 2 || k > INT32_MAX / 2) fail_arg("syrk: dimension too large");
     TimedRegion tr(sizeof(T) == 8 ? "syrk" : "ssyrk", s, (double)n * (double)(n + 1) * (double)k);
-    // FP64 takes 64 x 64 tiles at every size: the 128 x 128 FP64 tile holds
-    // one wave a SIMD and ran at 20.8 TF alone (n = 7936, k = 256) against
-    // 29.4 TF (40.5 against 28.8 at k = 512).  FP32 keeps 128 x 128 tiles for
-    // large outputs: with 64 x 64 its SYRK was faster but took the CUs from
-    // the factorisation's critical path (mixed Cholesky 34.5 against 33.5 ms).
-    int64_t big_tiles = cdiv(n, 128) * (cdiv(n, 128) + 1) / 2;
-    if (sizeof(T) == 4 && big_tiles >= 512)
-        launch_tile<T, 128, 128, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
-                                     ldc, 1);
-    else
-        launch_tile<T, 64, 64, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C,
-                                   ldc, 1, 1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
+    // 64 x 64 tiles at every size: the 128 x 128 FP64 tile holds one wave a
+    // SIMD and ran at 20.8 TF alone (n = 7936, k = 256) against 29.4 TF
+    // (40.5 against 28.8 at k = 512); FP32's lookahead SYRK beside the mixed
+    // Cholesky's diagonal kernels: 76.6 against 64.8 TF, 32.8 against 33.1 ms.
+    launch_tile<T, 64, 64, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C, ldc, 1,
+                               1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
 }
 
 }  // namespace
