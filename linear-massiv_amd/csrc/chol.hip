@@ -478,13 +478,17 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
         LMX_HIP(hipEventRecord(la.ready, s));
         LMX_HIP(hipStreamWaitEvent(la.aux, la.ready, 0));
         if (rest > narrow) {
-            if constexpr (sizeof(T) == 8) {
-                // At most two SYRK workgroups a CU (80 KB of LDS each): when
-                // one ends, the diagonal kernel (67 KB, 4 waves) fits in its
-                // place.  With three a CU it waited for the SYRK to drain,
-                // up to 300 us a block (n = 8192: 12.7 against 13.1 ms).
-                syrk_lower_sharing(la.aux, rest - narrow, kb, -1.0, A21 + narrow * lda, lda, 1.0,
-                                   A21 + narrow * lda + kb + narrow, lda, (size_t)80 * 1024);
+            if (!bf16) {
+                // Few enough SYRK workgroups a CU that when one ends, the
+                // diagonal kernel fits in its place: FP64 two a CU (80 KB
+                // of LDS each; the kernel takes 67 KB and 4 x 224
+                // registers), FP32 five (30 KB each; 34 KB and 4 x 128).
+                // As many as fit, they left it waiting for the SYRK to
+                // drain, up to 300 us a block (FP64 n = 8192: 12.7 against
+                // 13.1 ms; FP32 n = 16384: 32.0 against 32.9 ms).
+                const size_t fp = (size_t)(sizeof(T) == 8 ? 80 : 30) * 1024;
+                syrk_lower_sharing<T>(la.aux, rest - narrow, kb, T(-1), A21 + narrow * lda, lda, T(1),
+                                      A21 + narrow * lda + kb + narrow, lda, fp);
             } else {
                 chol_update<T>(la.aux, bf16, true, rest - narrow, rest - narrow, kb, A21 + narrow * lda, lda,
                                A21 + narrow * lda, A21 + narrow * lda + kb + narrow);

@@ -710,7 +710,8 @@ bool nn_ok(bool ta, bool tb, int64_t n, int64_t k, const T *A, int64_t lda, cons
 template <class T, int BM, int BN, int BK>
 void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, const T *A,
                  int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int lower = 0,
-                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}, size_t lds_footprint = 0) {
+                 int batch = 1, BatchStrides bs = BatchStrides{1, 0, 0, 0, 0, 0, 0}, size_t lds_footprint = 0,
+                 bool nn_sharing = false) {
     int tiles_m = (int)cdiv(m, BM), tiles_n = (int)cdiv(n, BN);
     dim3 grid(lower ? lower_tiles(tiles_m, tiles_n) : tiles_m * tiles_n, (unsigned)batch), block(256);
     // lds_footprint: the LDS one workgroup should hold in all (its tiles plus
@@ -718,7 +719,7 @@ void launch_tile(hipStream_t s, bool ta, bool tb, int m, int n, int k, T alpha, 
     // A GEMM that shares its CUs with FP32 panel workgroups keeps the first
     // kernel: the NN kernel is faster on its own but slows those panels more
     // (FP32 LU at n=16384: 155 against 151 ms).
-    const bool sharing = lds_footprint > 0 && sizeof(T) == 4;
+    const bool sharing = lds_footprint > 0 && sizeof(T) == 4 && !nn_sharing;
     if (!sharing && nn_ok<T>(ta, tb, n, k, A, lda, B, ldb, bs)) {
         const size_t st = gemm_nn_lds<T, BM, BN>(), reserve = lds_footprint > st ? lds_footprint - st : 0;
         if (tb)
@@ -853,7 +854,7 @@ void syrk_impl(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t
     // (40.5 against 28.8 at k = 512); FP32's lookahead SYRK beside the mixed
     // Cholesky's diagonal kernels: 76.6 against 64.8 TF, 32.8 against 33.1 ms.
     launch_tile<T, 64, 64, 16>(s, trans, !trans, (int)n, (int)n, (int)k, alpha, A, lda, A, lda, beta, C, ldc, 1,
-                               1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint);
+                               1, BatchStrides{1, 0, 0, 0, 0, 0, 0}, lds_footprint, true);
 }
 
 }  // namespace
@@ -880,10 +881,15 @@ void syrk_lower(hipStream_t s, int64_t n, int64_t k, double alpha, const double 
     syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc);
 }
 
-void syrk_lower_sharing(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
-                        double beta, double *C, int64_t ldc, size_t lds_footprint) {
-    syrk_impl<double>(s, n, k, alpha, A, lda, beta, C, ldc, false, lds_footprint);
+template <class T>
+void syrk_lower_sharing(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
+                        int64_t ldc, size_t lds_footprint) {
+    syrk_impl<T>(s, n, k, alpha, A, lda, beta, C, ldc, false, lds_footprint);
 }
+template void syrk_lower_sharing<double>(hipStream_t, int64_t, int64_t, double, const double *, int64_t, double,
+                                         double *, int64_t, size_t);
+template void syrk_lower_sharing<float>(hipStream_t, int64_t, int64_t, float, const float *, int64_t, float,
+                                        float *, int64_t, size_t);
 
 void syrk_lower_op(hipStream_t s, bool trans, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                    double beta, double *C, int64_t ldc) {

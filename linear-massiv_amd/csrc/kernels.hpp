@@ -61,8 +61,9 @@ void syrk_lower(hipStream_t s, int64_t n, int64_t k, float alpha, const float *A
 // column (the lower trapezoid), one launch.
 // syrk_lower with each workgroup of the 64 x 64 tiles holding lds_footprint
 // bytes of LDS (its tiles plus a reserve), so that fewer share a CU.
-void syrk_lower_sharing(hipStream_t s, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
-                        double beta, double *C, int64_t ldc, size_t lds_footprint);
+template <class T>
+void syrk_lower_sharing(hipStream_t s, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
+                        int64_t ldc, size_t lds_footprint);
 template <class T>
 void syrk_trap(hipStream_t s, int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, T beta, T *C,
                int64_t ldc);
