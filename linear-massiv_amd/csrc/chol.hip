@@ -464,17 +464,12 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
             gemm(s, false, false, rest, kb, kb, T(1), A21, lda, WT, BLK, T(0), X, kb);
             copy_block(s, X, kb, A21, lda, rest, kb);
         }
+        // The trailing SYRK may start once the panel is in place: it and
+        // the next block column's update below touch different elements.
+        // Recording `ready` here keeps the event markers out of the gap
+        // between that update and the next diagonal kernel.
         if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
         const int64_t narrow = std::min(nb, rest);
-        if (rest > 0 && !bf16) {
-            // the next block column, on and below its diagonal, in one launch
-            syrk_trap<T>(s, rest, narrow, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
-        } else if (rest > 0) {
-            chol_update<T>(s, bf16, true, narrow, narrow, kb, A21, lda, A21, A21 + kb);
-            if (rest > narrow)
-                chol_update<T>(s, bf16, false, rest - narrow, narrow, kb, A21 + narrow * lda, lda, A21,
-                               A21 + narrow * lda + kb);
-        }
         LMX_HIP(hipEventRecord(la.ready, s));
         LMX_HIP(hipStreamWaitEvent(la.aux, la.ready, 0));
         if (rest > narrow) {
@@ -496,6 +491,15 @@ void potrf_lookahead(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_inf
         }
         LMX_HIP(hipEventRecord(la.done, la.aux));
         pending = true;
+        if (rest > 0 && !bf16) {
+            // the next block column, on and below its diagonal, in one launch
+            syrk_trap<T>(s, rest, narrow, kb, T(-1), A21, lda, T(1), A21 + kb, lda);
+        } else if (rest > 0) {
+            chol_update<T>(s, bf16, true, narrow, narrow, kb, A21, lda, A21, A21 + kb);
+            if (rest > narrow)
+                chol_update<T>(s, bf16, false, rest - narrow, narrow, kb, A21 + narrow * lda, lda, A21,
+                               A21 + narrow * lda + kb);
+        }
     }
     if (pending) LMX_HIP(hipStreamWaitEvent(s, la.done, 0));
 }
