@@ -1295,8 +1295,8 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
     // own outer panel at once and the trailing columns once the auxiliary
     // stream is done with them; the columns of earlier panels (L) get them
     // on the auxiliary stream, after the GEMM that reads them.
-    //   stream s:   panels(b); wait F(b-1); swaps(b) on [k0+nbw, N); W = L11^-1;
-    //               U12 and A22 -= L21 U12 on panel b+1's columns; record E(b)
+    //   stream s:   panels(b); W = L11^-1; wait F(b-1); swaps(b) on [k0+nbw, N);
+    //               record E(b); U12 and A22 -= L21 U12 on panel b+1's columns
     //   stream aux: wait E(b); swaps(b) on [0, k0); U12 and A22 -= L21 U12 on
     //               the columns beyond panel b+1; record F(b)
     AuxStream la;
@@ -1363,14 +1363,18 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
         // stream would otherwise spend waiting for the previous GEMM
         if (trail > 0) tri_inverse_build<T>(s, true, false, true, nbw, A11, lda, W);
         if (pending) LMX_HIP(hipStreamWaitEvent(s, evF, 0));
+        // The auxiliary stream's part needs the row moves on the trailing
+        // columns and W, not the next columns' solve and update, which touch
+        // other columns: E goes before those, so that no event marker sits
+        // between them and the next panel.
+        if (trail > 0) gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
+        LMX_HIP(hipEventRecord(evE, s));
+        LMX_HIP(hipStreamWaitEvent(aux, evE, 0));
         if (trail > 0) {
-            gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
             tri_inverse_solve<T>(s, true, false, nbw, narrow, A11, lda, W, A12, lda);
             trailing_update<T>(s, bf16, trail, narrow, nbw, A11 + (int64_t)nbw * lda, lda, A12,
                                A12 + (int64_t)nbw * lda, false);
         }
-        LMX_HIP(hipEventRecord(evE, s));
-        LMX_HIP(hipStreamWaitEvent(aux, evE, 0));
         gather_rows_deferred<T>(aux, A, lda, 0, k0, pairs, np);
         if (trail > narrow) {
             tri_inverse_solve<T>(aux, true, false, nbw, trail - narrow, A11, lda, W, A12 + narrow, lda);
