@@ -38,7 +38,8 @@ KERNEL_FAMILIES = ["gemm", "sgemm", "gemm64", "sgemm64", "syrk", "ssyrk", "bgemm
 # HBM bytes per launch of a workload's dominant kernel, from the rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
 # (workload, n, kernel family) -> file
-TRAFFIC_PROFILES = {("lu_mixed", 16384, "sgemm"): "sgemm_lu_mixed_n16384_traffic.md"}
+TRAFFIC_PROFILES = {("lu_mixed", 16384, "sgemm"): "sgemm_lu_mixed_n16384_traffic.md",
+                    ("chol_fp64", 8192, "syrk"): "dsyrk_chol_fp64_n8192_traffic.md"}
 
 
 def profiled_traffic(workload, n, family):
