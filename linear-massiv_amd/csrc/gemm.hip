@@ -72,9 +72,12 @@ __host__ __device__ inline int lower_tiles(int tiles_m, int tiles_n) {
 }
 
 __device__ inline void tile_of(int tiles_m, int tiles_n, bool lower, int &tm, int &tn) {
-    int nwg = lower ? lower_tiles(tiles_m, tiles_n) : tiles_m * tiles_n;
-    int bid = blockIdx.x;
-    if ((nwg & 7) == 0) bid = (bid & 7) * (nwg >> 3) + (bid >> 3);
+    const int nwg = lower ? lower_tiles(tiles_m, tiles_n) : tiles_m * tiles_n;
+    // workgroup b runs on XCD b mod 8: XCD x takes the x-th contiguous range
+    // of the enumeration (ranges of q + 1 tiles for the first nwg mod 8 XCDs,
+    // of q for the others)
+    const int q = nwg >> 3, r = nwg & 7, x = (int)blockIdx.x & 7, j = (int)blockIdx.x >> 3;
+    const int bid = x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
     if (lower) {
         const int tri = tiles_n * (tiles_n + 1) / 2;
         if (bid >= tri) {
