@@ -39,7 +39,9 @@ KERNEL_FAMILIES = ["gemm", "sgemm", "gemm64", "sgemm64", "syrk", "ssyrk", "bgemm
 # FETCH_SIZE / WRITE_SIZE passes summarised in profiles/
 # (workload, n, kernel family) -> file
 TRAFFIC_PROFILES = {("lu_mixed", 16384, "sgemm"): "sgemm_lu_mixed_n16384_traffic.md",
-                    ("chol_fp64", 8192, "syrk"): "dsyrk_chol_fp64_n8192_traffic.md"}
+                    ("chol_fp64", 8192, "syrk"): "dsyrk_chol_fp64_n8192_traffic.md",
+                    ("lu_fp64", 8192, "gemm"): "dgemm_lu_fp64_n8192_traffic.md",
+                    ("chol_mixed", 16384, "ssyrk"): "ssyrk_chol_mixed_n16384_traffic.md"}
 
 
 def profiled_traffic(workload, n, family):
