@@ -1089,6 +1089,67 @@ __global__ __launch_bounds__(256) void row_gather_multi_kernel(T *__restrict__ X
     }
 }
 
+// The same moves for rows [r0, r0 + nr), nr <= GATHER_ROWS, with the
+// sub-panels' permutations composed first: M (LDS) maps each row to the
+// row its final value comes from, built move list by move list (all of a
+// list's sources read before its destinations are written, as the moves
+// themselves); then every destination row takes its source's value in one
+// load pass and one store pass, instead of a load, store and barrier round
+// trip per sub-panel.  A workgroup takes 16 columns; thread t column t % 16
+// and destinations t / 16, t / 16 + 16, ....  A row listed as destination
+// in several lists is written several times with the same value.
+constexpr int GATHER_ROWS = 16384;
+constexpr int GATHER_COLS = 16;
+template <class T>
+__global__ __launch_bounds__(256) void row_gather_composed_kernel(T *__restrict__ X, int64_t ldx, int c0, int c1,
+                                                                  int r0, int nr, const int32_t *__restrict__ pairs,
+                                                                  int npanels) {
+    critical_path_priority();
+    constexpr int MAXM = 2 * KB, MAXD = PANELS_PER_BLOCK * MAXM, PER = MAXD / (256 / GATHER_COLS);
+    __shared__ int M[GATHER_ROWS];
+    __shared__ int sd[MAXD];
+    const int t = threadIdx.x;
+    for (int r = t; r < nr; r += 256) M[r] = r0 + r;
+    __syncthreads();
+    int tot = 0;
+    for (int p = 0; p < npanels; ++p) {
+        const int32_t *pp = pairs + (int64_t)p * PAIR_STRIDE;
+        const int cnt = min(pp[0], MAXM);
+        int v = 0, d = 0;
+        if (t < cnt) {
+            d = pp[1 + t];
+            v = M[pp[1 + MAXM + t] - r0];
+        }
+        __syncthreads();
+        if (t < cnt) {
+            M[d - r0] = v;
+            sd[tot + t] = d;
+        }
+        tot += cnt;
+        __syncthreads();
+    }
+    const int cl = t % GATHER_COLS, g = t / GATHER_COLS;
+    const int col = c0 + (int)blockIdx.x * GATHER_COLS + cl;
+    const int cc = min(col, c1 - 1);
+    T val[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = g + i * (256 / GATHER_COLS);
+        const int src = e < tot ? M[sd[e] - r0] : r0;
+        val[i] = X[(int64_t)src * ldx + cc];
+    }
+    // every load has returned before any thread stores (a destination can
+    // be another entry's source)
+#pragma unroll
+    for (int i = 0; i < PER; ++i) asm volatile("" : "+v"(val[i]));
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = g + i * (256 / GATHER_COLS);
+        if (col < c1 && e < tot) X[(int64_t)sd[e] * ldx + col] = val[i];
+    }
+}
+
 __global__ void iota_kernel(int32_t *p, int n) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = i;
@@ -1216,12 +1277,21 @@ void gather_rows(hipStream_t s, T *X, int64_t ldx, int c0, int c1, int c2, int c
     LMX_LAUNCH_CHECK();
 }
 
+// The deferred moves of npanels sub-panels whose rows lie in [r0, r1);
+// `composed` (on the critical path) takes the composed-permutation kernel,
+// whose 66 KB of LDS a workgroup would otherwise crowd the panels' CUs.
 template <class T>
-void gather_rows_deferred(hipStream_t s, T *A, int64_t lda, int c0, int c1, const int32_t *pairs, int npanels) {
+void gather_rows_deferred(hipStream_t s, T *A, int64_t lda, int c0, int c1, int r0, int r1, const int32_t *pairs,
+                          int npanels, bool composed) {
     if (c1 <= c0 || npanels <= 0) return;
     TimedRegion tr("laswp", s);
-    hipLaunchKernelGGL(row_gather_multi_kernel<T>, dim3((unsigned)cdiv(c1 - c0, 64)), dim3(256), 0, s, A, lda, c0, c1,
-                       pairs, npanels);
+    if (composed && r1 - r0 <= GATHER_ROWS && npanels <= PANELS_PER_BLOCK) {
+        hipLaunchKernelGGL(row_gather_composed_kernel<T>, dim3((unsigned)cdiv(c1 - c0, GATHER_COLS)), dim3(256), 0, s,
+                           A, lda, c0, c1, r0, r1 - r0, pairs, npanels);
+    } else {
+        hipLaunchKernelGGL(row_gather_multi_kernel<T>, dim3((unsigned)cdiv(c1 - c0, 64)), dim3(256), 0, s, A, lda, c0,
+                           c1, pairs, npanels);
+    }
     LMX_LAUNCH_CHECK();
 }
 
@@ -1367,7 +1437,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
         // columns and W, not the next columns' solve and update, which touch
         // other columns: E goes before those, so that no event marker sits
         // between them and the next panel.
-        if (trail > 0) gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, pairs, np);
+        if (trail > 0) gather_rows_deferred<T>(s, A, lda, k0 + nbw, N, k0, N, pairs, np, true);
         LMX_HIP(hipEventRecord(evE, s));
         LMX_HIP(hipStreamWaitEvent(aux, evE, 0));
         if (trail > 0) {
@@ -1375,7 +1445,7 @@ void getrf_impl(hipStream_t s, int64_t n, T *A, int64_t lda, int32_t *d_ipiv, in
             trailing_update<T>(s, bf16, trail, narrow, nbw, A11 + (int64_t)nbw * lda, lda, A12,
                                A12 + (int64_t)nbw * lda, false);
         }
-        gather_rows_deferred<T>(aux, A, lda, 0, k0, pairs, np);
+        gather_rows_deferred<T>(aux, A, lda, 0, k0, k0, N, pairs, np, false);
         if (trail > narrow) {
             tri_inverse_solve<T>(aux, true, false, nbw, trail - narrow, A11, lda, W, A12 + narrow, lda);
             // at most two GEMM workgroups per CU, so every CU keeps room for
