@@ -381,6 +381,27 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
         }
     };
 
+    // The 64 x 64 tile reads its C values (beta != 0) before the k loop, at
+    // addresses clamped into C: with a short k (the factorisations' updates
+    // have k = 32 or 256) the epilogue's reads otherwise wait on memory after
+    // the last MFMA.  16 values a lane.
+    constexpr bool PREC = BM == 64 && BN == 64;
+    T cpre[PREC ? MT : 1][PREC ? NT : 1][4];
+    if constexpr (PREC) {
+        if (beta != T(0)) {
+#pragma unroll
+            for (int a = 0; a < MT; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int gi = min(m0 + wm0 + a * 16 + acc_row<T>(lk, r), m - 1);
+                        const int gj = min(n0 + wn0 + b * 16 + lr, n - 1);
+                        cpre[a][b][r] = C[(int64_t)gi * ldc + gj];
+                    }
+        }
+    }
+
     const int KT = (k + BK - 1) / BK;
     if (KT > 0) {
         load(0);
@@ -434,7 +455,11 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int m, int n, int k, T alp
                 if (gi < m && (!lower || gj <= gi)) {
                     T *c = C + (int64_t)gi * ldc + gj;
                     T v = alpha * acc[a][b][r];
-                    if (beta != T(0)) v += beta * *c;
+                    if constexpr (PREC) {
+                        if (beta != T(0)) v += beta * cpre[a][b][r];
+                    } else {
+                        if (beta != T(0)) v += beta * *c;
+                    }
                     *c = v;
                 }
             }
