@@ -174,8 +174,9 @@ int lmx_dgesv_dev(void *stream, int64_t n, int64_t nrhs, double *dA,
 
 /* ---- Cholesky ----------------------------------------------------------- */
 
-/* In-place lower Cholesky A = G G^T reading the lower triangle only; the
- * strict upper triangle is left as given.  Replaces the factorisation of
+/* In-place lower Cholesky A = G G^T from the lower triangle only (GEMM
+ * tiles on the diagonal may load the upper elements they cover, and never
+ * use them); the strict upper triangle is left as given.  Replaces the factorisation of
  * choleskySolveP (src/.../Solve/Cholesky.hs:246-256, panelCholFactor
  * :279-321) and cholesky (:114-147, whose upper triangle is zero: the
  * binding zeroes it). */

@@ -1,5 +1,7 @@
 // Lower Cholesky factorisation (right-looking, blocked, with lookahead) and
-// solve, for row-major matrices.  Only the lower triangle is read or written.
+// solve, for row-major matrices.  Only the lower triangle is written, and
+// the results depend on it alone (the GEMM tiles on the diagonal load the
+// upper elements they cover and never use them).
 //
 // For each block column of width 256:
 //   - the diagonal block and its inverse (diag_block), by halves down to
