@@ -235,6 +235,26 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
         }
     };
 
+    // C values read before the k loop (beta != 0), as in the NN kernel: all
+    // of a 64 x 64 tile's, the first half of the rows of an FP32 128 x 128
+    // tile's (32 more registers keep it within two waves a SIMD)
+    constexpr int PH = (BM == 64 && BN == 64) ? MT : ((sizeof(T) == 4 && BM == 128 && BN == 128) ? MT / 2 : 0);
+    T cpre[PH > 0 ? PH : 1][NT][4];
+    if constexpr (PH > 0) {
+        if (beta != T(0)) {
+#pragma unroll
+            for (int a = 0; a < PH; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int gi = min(m0 + wm0 + a * 16 + acc_row<T>(lk, r), m - 1);
+                        const int gj = min(n0 + wn0 + b * 16 + lr, n - 1);
+                        cpre[a][b][r] = C[(int64_t)gi * ldc + gj];
+                    }
+        }
+    }
+
     const int KT = (k + BK - 1) / BK;
     if (KT > 0) {
         load(0);
@@ -274,7 +294,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int m, int n, int k, T a
                 if (gi < m && (!lower || gj <= gi)) {
                     T *c = C + (int64_t)gi * ldc + gj;
                     T v = alpha * acc[a][b][r];
-                    if (beta != T(0)) v += beta * *c;
+                    if (beta != T(0)) v += beta * (a < PH ? cpre[a < PH ? a : 0][b][r] : *c);
                     *c = v;
                 }
             }
